@@ -1,0 +1,182 @@
+"""Pins of the FP64 oracle (oracle/collision.py) — CPU only.
+
+Each check is chosen so that a plausible mistake in the restated
+reference algorithm (inelastic_collisions.py:57-137, 208-230) fails:
+brute force catches FFT sequencing/normalisation/index-order mistakes,
+BKW catches kernel-normalisation and loss-term mistakes, Haff catches a
+wrong restitution factor a_j, the invariants catch batch/axis mix-ups.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import brute
+from oracle import collision as C
+from oracle import quadrature as Qd
+
+S_C1 = 2.718491035932  # L = 6
+S_C3 = 3.624654714576  # L = 8
+
+
+def _two_gauss(op, seed=0, T=0.5):
+    rng = np.random.default_rng(seed)
+    v = op.v_grids()
+    c1 = np.array([1.0, 0.0] + [0.0] * (op.d - 2)) + rng.normal(0, 0.1, op.d)
+    c2 = -np.array([1.0, 0.0] + [0.0] * (op.d - 2)) + rng.normal(0, 0.1, op.d)
+
+    def g(c):
+        return np.exp(-sum((v[a] - c[a]) ** 2 for a in range(op.d)) / (2 * T)) / (2 * math.pi * T) ** (op.d / 2)
+
+    f = 0.5 * (g(c1) + g(c2))
+    return f / (f.sum() * op.delta ** op.d)
+
+
+@pytest.mark.parametrize("d,N,nr,M", [(2, 8, 4, 5), (3, 4, 3, 14)])
+def test_brute_force_matches(d, N, nr, M):
+    rng = np.random.default_rng(1)
+    if d == 2:
+        sig, ws = Qd.circle_rule(M)
+        ws = ws * (1 + 0.1 * np.arange(M))  # non-uniform weights exercise the per-node path
+    else:
+        sig, ws = Qd.lebedev_rule(M)
+    op = C.OracleOperator(d, N, 2.0, nr, sig, ws, e=0.6, e0=0.95, gamma=1.0)
+    f = rng.random((N,) * d)
+    gp, gl = op.gain_loss(f)
+    bg, bl = brute.brute_collide(f, d, N, op.L, op.R, op.r, op.wr, op.sigma, op.wsigma, 0.6, 0.95, 1.0)
+    assert np.abs(gp - bg).max() <= 1e-13 * np.abs(bg).max()
+    assert np.abs(gl - bl).max() <= 1e-13 * np.abs(bl).max()
+
+
+def _bkw2(v2, K):
+    return 1 / (2 * math.pi * K * K) * np.exp(-v2 / (2 * K)) * (2 * K - 1 + (1 - K) * v2 / (2 * K))
+
+
+def test_bkw_2d_elastic_maxwell():
+    # B = 1/(2 pi), K = 1 - exp(-t/8)/2 ; Q = df/dt
+    sig, ws = Qd.circle_rule(8)
+    op = C.OracleOperator(2, 32, S_C1, 32, sig, ws, e=1.0, e0=1.0, gamma=0.0)
+    v = op.v_grids()
+    v2 = v[0] ** 2 + v[1] ** 2
+    t = 1.0
+    K = 1 - math.exp(-t / 8) / 2
+    h = 1e-5
+    dfdt = (_bkw2(v2, K + h) - _bkw2(v2, K - h)) / (2 * h) * math.exp(-t / 8) / 16
+    Q = op.collide(_bkw2(v2, K), with_loss=True)
+    err = np.abs(Q - dfdt).max() / np.abs(dfdt).max()
+    assert err < 1e-4, err
+
+
+def test_bkw_3d_elastic_maxwell():
+    # B = 1/(4 pi), K = 1 - exp(-t/6)
+    sig, ws = Qd.lebedev_rule(38)
+    op = C.OracleOperator(3, 24, 3.0, 24, sig, ws, e=1.0, e0=1.0, gamma=0.0)
+    v = op.v_grids()
+    v2 = sum(x * x for x in v)
+
+    def bkw(K):
+        return (2 * math.pi * K) ** -1.5 * np.exp(-v2 / (2 * K)) * ((5 * K - 3) / (2 * K) + (1 - K) * v2 / (2 * K * K))
+
+    t = 5.5
+    K = 1 - math.exp(-t / 6)
+    h = 1e-5
+    dfdt = (bkw(K + h) - bkw(K - h)) / (2 * h) * math.exp(-t / 6) / 6
+    Q = op.collide(bkw(K), with_loss=True)
+    err = np.abs(Q - dfdt).max() / np.abs(dfdt).max()
+    assert err < 1e-3, err
+
+
+@pytest.mark.parametrize("M,tol", [(8, 0.08), (64, 3e-3)])
+def test_haff_energy_identity(M, tol):
+    # inelastic Maxwell molecules: dT/dt = -(1 - e^2) rho T / 4 (any d);
+    # reference boltzmann.py:103-105 encodes the same rate.
+    e = 0.8
+    sig, ws = Qd.circle_rule(M)
+    op = C.OracleOperator(2, 32, S_C1, 32, sig, ws, e=e, e0=e, gamma=0.0)
+    f = _two_gauss(op)
+    v, dv = op.v_grids(), op.delta
+    Q = op.collide(f, with_loss=True)
+    rho, _, _ = C.moments(f, v, dv)
+    _, qm, qe = C.moments(Q, v, dv)
+    T = C.temperature(f, v, dv)
+    rate = qe / rho  # dT/dt = 2 dE/dt / (d rho) with d = 2 and momentum conserved
+    assert abs(rate / (-(1 - e * e) * rho * T / 4) - 1) < tol
+
+
+def test_maxwellian_elastic_equilibrium():
+    sig, ws = Qd.circle_rule(16)
+    op = C.OracleOperator(2, 32, S_C1, 32, sig, ws, e=1.0, e0=1.0, gamma=1.0)
+    v = op.v_grids()
+    m = np.exp(-(v[0] ** 2 + v[1] ** 2) / 2) / (2 * math.pi)
+    assert np.abs(op.collide(m, with_loss=True)).max() < 1e-5 * m.max()
+
+
+def test_invariants_scaling_shift_batch():
+    sig, ws = Qd.circle_rule(8)
+    op = C.OracleOperator(2, 16, 2.0, 8, sig, ws, e=0.7, e0=0.9, gamma=1.0)
+    rng = np.random.default_rng(3)
+    f = rng.random((16, 16))
+    q = op.collide(f, with_loss=True)
+    assert np.allclose(op.collide(2.5 * f, with_loss=True), 6.25 * q, rtol=0, atol=1e-13 * np.abs(q).max() * 6.25)
+    fr = np.roll(f, (3, -5), axis=(0, 1))
+    qr = op.collide(fr, with_loss=True)
+    assert np.abs(qr - np.roll(q, (3, -5), axis=(0, 1))).max() < 1e-13 * np.abs(q).max()
+    fb = np.stack([f, 2 * f, fr])
+    qb = op.collide(fb, with_loss=True)
+    assert np.abs(qb[0] - q).max() < 1e-14 * np.abs(q).max()
+    assert np.abs(qb[2] - qr).max() < 1e-14 * np.abs(q).max()
+
+
+def test_gain_only_default_and_heat_bath():
+    sig, ws = Qd.circle_rule(8)
+    op = C.OracleOperator(2, 16, 2.0, 8, sig, ws, e=0.7, e0=0.7)
+    f = np.random.default_rng(4).random((16, 16))
+    gp, gl = op.gain_loss(f)
+    assert np.array_equal(op.collide(f), gp)
+    assert np.allclose(op.collide(f, with_loss=True), gp - gl, atol=1e-14)
+    # Laplacian of a periodic plane wave
+    v = op.v_grids()
+    k = 3
+    w = np.cos(math.pi * k * (v[0] + op.L) / op.L)
+    lap = op.laplacian(w)
+    assert np.abs(lap + (math.pi * k / op.L) ** 2 * w).max() < 1e-12
+    out = op(f, heat_bath=0.3)
+    assert np.allclose(out, gp + 0.3 * op.laplacian(f), atol=1e-14)
+
+
+def test_projection_conserves_to_roundoff():
+    sig, ws = Qd.lebedev_rule(14)
+    op = C.OracleOperator(3, 8, 2.0, 4, sig, ws, e=0.5, e0=0.5, gamma=1.0)
+    f = np.random.default_rng(5).random((2, 8, 8, 8))
+    q = op.collide(f, with_loss=True)
+    qc = C.project_conservative(q, op.v_grids())
+    v = op.v_grids()
+    scale = np.abs(q).sum()
+    assert abs(qc.sum()) < 1e-14 * scale
+    for a in range(3):
+        assert abs((qc * v[a]).sum()) < 1e-14 * scale * op.L
+    # energy is left free: projection of an energy-only perturbation keeps it
+    assert np.abs(C.project_conservative(qc, v) - qc).max() < 1e-15 * np.abs(q).max() * 10
+
+
+def test_mass_momentum_quadrature_accuracy():
+    sig, ws = Qd.circle_rule(64)
+    op = C.OracleOperator(2, 32, S_C1, 32, sig, ws, e=0.8, e0=0.8)
+    f = _two_gauss(op, seed=7)
+    q = op.collide(f, with_loss=True)
+    rho, mom, _ = C.moments(q, op.v_grids(), op.delta)
+    assert abs(rho) < 1e-7 and np.abs(mom).max() < 1e-7, (rho, mom)
+
+
+def test_planted_mistake_is_caught(monkeypatch):
+    """Plant a sign error in the translate kernel; the BKW pin must fail."""
+    sig, ws = Qd.circle_rule(8)
+    op = C.OracleOperator(2, 32, S_C1, 32, sig, ws, e=1.0, e0=1.0, gamma=0.0)
+    op.exp = np.conj(op.exp)
+    v = op.v_grids()
+    v2 = v[0] ** 2 + v[1] ** 2
+    K = 1 - math.exp(-1 / 8) / 2
+    h = 1e-5
+    dfdt = (_bkw2(v2, K + h) - _bkw2(v2, K - h)) / (2 * h) * math.exp(-1 / 8) / 16
+    Q = op.collide(_bkw2(v2, K), with_loss=True)
+    assert np.abs(Q - dfdt).max() / np.abs(dfdt).max() > 1e-2
