@@ -1,0 +1,116 @@
+/* C ABI of the MI355X (gfx950) fast spectral inelastic collision library.
+ *
+ * Every entry point takes plain pointers and sizes.  Device pointers are HIP
+ * device memory owned by the caller (e.g. a torch tensor's data_ptr()); the
+ * `stream` argument is a hipStream_t (NULL = the null stream).  Every call
+ * returns IB_OK (0) or a negative status; ib_last_error() gives the message
+ * of the last failure on the calling thread.
+ *
+ * Citations are to the reference interface each entry point replaces
+ * (paths relative to the reference repository).
+ */
+#ifndef INELASTIC_H_
+#define INELASTIC_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IB_OK 0
+#define IB_ERR_ARG -1       /* ValueError in the reference          */
+#define IB_ERR_HIP -2       /* HIP runtime failure                  */
+#define IB_ERR_UNSUPPORTED -3 /* NotImplementedError in the reference */
+#define IB_ERR_BC -4        /* missing boundary condition (Exception) */
+
+#define IB_PREC_FP64 0      /* plain FP64 form: C2C, staged batched FFTs        */
+#define IB_PREC_FP32 1      /* FP32 fast form: fused, Kahan-compensated sums    */
+
+typedef struct ib_plan ib_plan;
+
+/* Operator description.  Replaces the state built by
+ * src/collision/spectral_mesh.py:8-66 (SpectralMesh) and
+ * src/collision/inelastic_collisions.py:30-52 (load_parameters).
+ * r/wr: radial Gauss-Legendre nodes on [0, R]; sigma: (m, dim) unit vectors;
+ * wsigma: (m,) weights; e_r: (n_param, nr) restitution profile
+ * e_j = (e0 - e)/2 tanh(r_j - R/2) + (e0 + e)/2 per parameter set. */
+typedef struct {
+  int dim;        /* 2 or 3 */
+  int n;          /* points per velocity axis: 8, 16 or 32 */
+  int nr;         /* radial nodes */
+  int m;          /* circle / sphere nodes */
+  double L;       /* velocity box half width */
+  double gamma;   /* VHS exponent */
+  int precision;  /* IB_PREC_FP64 or IB_PREC_FP32 */
+  int n_param;    /* number of restitution parameter sets (>= 1) */
+  const double* r;
+  const double* wr;
+  const double* sigma;
+  const double* wsigma;
+  const double* e_r;
+} ib_plan_desc;
+
+/* Precompute the kernel tables on the device.
+ * Replaces src/collision/inelastic_collisions.py:57-137 (perform_precomputation). */
+int ib_plan_create(const ib_plan_desc* desc, ib_plan** out);
+
+/* Replace the restitution parameter sets (n_param x nr profile) of a plan.
+ * Replaces re-running inelastic_collisions.py:72-103 for a new e. */
+int ib_plan_set_params(ib_plan* plan, int n_param, const double* e_r, void* stream);
+
+int ib_plan_destroy(ib_plan* plan);
+
+/* Q(f, f) for `batch` cells: f, q are (batch, n^dim) float64 device arrays.
+ * param_idx: device int32 (batch,) parameter-set index per cell, or NULL for 0.
+ * with_loss = 0 returns the gain term only (the reference default);
+ * heat_bath != 0 adds heat_bath * Laplacian(f).
+ * Replaces src/collision/base.py:32-58 (__call__) and
+ * src/collision/inelastic_collisions.py:208-227 (collide), :229-230 (laplacian). */
+int ib_collide(ib_plan* plan, const double* f, double* q, long long batch,
+               const int* param_idx, int with_loss, double heat_bath, void* stream);
+
+/* Laplacian of f (spectral), out = Re ifft(-pi^2 |k|^2 / L^2 fft f).
+ * Replaces src/collision/inelastic_collisions.py:229-230. */
+int ib_laplacian(ib_plan* plan, const double* f, double* out, long long batch, void* stream);
+
+/* FP64 moments per cell: out (batch, dim + 2) = [rho, rho u_1..u_d (momentum), energy],
+ * each times dv^dim, v_i = -L + (i + 1/2) dv.  Replaces src/state.py:31-52 (2D only there). */
+int ib_moments(const double* f, double* out, long long batch, int dim, int n, double L,
+               void* stream);
+
+/* In-place FP64 L2 projection of q (batch, n^dim) onto {sum q = 0, sum q v_a = 0}.
+ * New (the reference has no conservative correction). */
+int ib_project_conservative(double* q, long long batch, int dim, int n, double L, void* stream);
+
+/* Second-order TVD wave-propagation increment of v_x d/dx f on a 1D grid.
+ * f, df: (nx, nv) float64 with nv velocity points per cell; v1: (nv,) advection speed
+ * per velocity point; dtdx = dt / dx.  order: 1 or 2; limiter: 0 minmod, 1 MC;
+ * bc: 1 extrapolation, 2 periodic.  vx_rows: the leading velocity extent (N), used to
+ * reproduce the reference's s[LL-1:UL-1] slice check (nx + 2 >= vx_rows).
+ * Replaces src/boltzmann.py:177-205 (df_advection) with src/solver.py:51-105
+ * (ghost cells), src/riemann/advection_1D.py:4-21 and src/limiters/tvd.py:6-40. */
+int ib_tvd1d(const double* f, double* df, int nx, long long nv, const double* v1,
+             double dtdx, int order, int limiter, int bc_lower, int bc_upper, int vx_rows,
+             void* stream);
+
+/* y <- y + a x over n float64 values.  Replaces the `state.f += M * df` updates of
+ * src/boltzmann.py:84,126-128,137-140,153-158. */
+int ib_axpy(double* y, const double* x, double a, long long n, void* stream);
+
+/* out <- adv + (dt * q) / kn (adv may be NULL for 0).  Replaces src/boltzmann.py:160-164 (df). */
+int ib_df_combine(const double* adv, const double* q, double* out, double dt, double kn,
+                  long long n, void* stream);
+
+/* Host-side evaluation of the device special functions (no GPU needed), for tests:
+ * dim = 2 -> J0(x), dim = 3 -> sin(x)/x. */
+double ib_host_sp(int dim, double x);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* ib_last_error(void);
+
+/* Library version string. */
+const char* ib_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* INELASTIC_H_ */

@@ -1,0 +1,286 @@
+// C ABI (include/inelastic.h): argument checks, plan lifetime, dispatch.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "common.h"
+#include "plan.h"
+
+namespace ib {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+int DevBuf::ensure(size_t want) {
+  if (want <= bytes) return IB_OK;
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+  IB_HIP(hipMalloc(&ptr, want));
+  bytes = want;
+  return IB_OK;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+int collide_plain(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                  int with_loss, double hb, int no_gain, hipStream_t st);
+int collide_fused(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                  int with_loss, double hb, hipStream_t st, int* handled);
+int launch_tables(Plan* P, const double* d_wr, hipStream_t st);
+int launch_tvd1d(const double* f, double* df, int nx, long long nv, const double* v1, double dtdx,
+                 int order, int limiter, int bc_lo, int bc_hi, hipStream_t st);
+int launch_axpy(double* y, const double* x, double a, long long n, hipStream_t st);
+int launch_df_combine(const double* adv, const double* q, double* out, double dt, double kn,
+                      long long n, hipStream_t st);
+int launch_moments(const double* f, double* out, long long batch, int dim, int n, double L,
+                   hipStream_t st);
+int launch_project(double* q, long long batch, int dim, int n, double L, hipStream_t st);
+
+template <typename X>
+static int upload(X** dst, const X* src, size_t count) {
+  IB_HIP(hipMalloc((void**)dst, count * sizeof(X)));
+  IB_HIP(hipMemcpy(*dst, src, count * sizeof(X), hipMemcpyHostToDevice));
+  return IB_OK;
+}
+
+static void free_param_tables(Plan* p) {
+  if (p->d_a) (void)hipFree(p->d_a);
+  if (p->d_spg) (void)hipFree(p->d_spg);
+  p->d_a = nullptr;
+  p->d_spg = nullptr;
+}
+
+// a_j = pi (1 + e_j) / (4 L) per parameter set (inelastic_collisions.py:80),
+// then the sp(a_j r_j |k|) table on the device.
+static int set_params(Plan* p, int n_param, const double* e_r, hipStream_t st, const double* d_wr) {
+  free_param_tables(p);
+  p->n_param = n_param;
+  p->e_r.assign(e_r, e_r + (size_t)n_param * p->nr);
+  std::vector<double> a((size_t)n_param * p->nr);
+  for (size_t i = 0; i < a.size(); ++i) a[i] = 0.25 * M_PI * (1 + p->e_r[i]) / p->L;
+  int rc = upload(&p->d_a, a.data(), a.size());
+  if (rc) return rc;
+  const size_t tsz = p->precision == IB_PREC_FP64 ? sizeof(double) : sizeof(float);
+  IB_HIP(hipMalloc(&p->d_spg, (size_t)n_param * p->nr * p->nq * tsz));
+  if (!p->d_lam) IB_HIP(hipMalloc(&p->d_lam, (size_t)p->nq * tsz));
+  rc = launch_tables(p, d_wr, st);
+  if (rc) return rc;
+  return IB_OK;
+}
+
+}  // namespace ib
+
+using namespace ib;
+
+struct ib_plan : public Plan {
+  double* d_wr = nullptr;
+};
+
+extern "C" {
+
+const char* ib_last_error(void) { return g_err.c_str(); }
+
+const char* ib_version(void) { return "inelastic-mi355x 0.1 (gfx950)"; }
+
+double ib_host_sp(int dim, double x) { return sp_func(dim, x); }
+
+int ib_plan_create(const ib_plan_desc* d, ib_plan** out) {
+  if (!d || !out) {
+    set_error("null argument");
+    return IB_ERR_ARG;
+  }
+  *out = nullptr;
+  if (d->dim != 2 && d->dim != 3) {
+    set_error("Dimension must be 2 or 3.");
+    return IB_ERR_ARG;
+  }
+  if (d->n != 8 && d->n != 16 && d->n != 32) {
+    set_error("n must be 8, 16 or 32");
+    return IB_ERR_UNSUPPORTED;
+  }
+  if (d->nr < 1 || d->m < 1 || d->n_param < 1 || !(d->L > 0)) {
+    set_error("nr, m, n_param must be >= 1 and L > 0");
+    return IB_ERR_ARG;
+  }
+  if (d->precision != IB_PREC_FP64 && d->precision != IB_PREC_FP32) {
+    set_error("precision must be IB_PREC_FP64 or IB_PREC_FP32");
+    return IB_ERR_ARG;
+  }
+  if (!d->r || !d->wr || !d->sigma || !d->wsigma || !d->e_r) {
+    set_error("null quadrature array");
+    return IB_ERR_ARG;
+  }
+  ib_plan* p = new ib_plan();
+  p->dim = d->dim;
+  p->n = d->n;
+  p->nr = d->nr;
+  p->m = d->m;
+  p->L = d->L;
+  p->gamma = d->gamma;
+  p->precision = d->precision;
+  p->npts = 1;
+  for (int a = 0; a < p->dim; ++a) p->npts *= p->n;
+  p->nq = p->dim * (p->n / 2) * (p->n / 2) + 1;
+  p->r.assign(d->r, d->r + d->nr);
+  p->wr.assign(d->wr, d->wr + d->nr);
+  p->sigma.assign(d->sigma, d->sigma + (size_t)d->m * d->dim);
+  p->wsigma.assign(d->wsigma, d->wsigma + d->m);
+  p->sphere_area = 2 * std::pow(M_PI, 0.5 * p->dim) / std::tgamma(0.5 * p->dim);
+  // gain weight r^(gamma+d-1) w_r w_sigma: the reference's |S| r^(gamma+d-1) w_r w_sigma
+  // (inelastic_collisions.py:97-103) divided by |S| (:225-227)
+  std::vector<double> wjm((size_t)p->nr * p->m);
+  for (int j = 0; j < p->nr; ++j)
+    for (int m = 0; m < p->m; ++m) {
+      const double pre = p->sphere_area * std::pow(p->r[j], p->gamma + p->dim - 1);
+      wjm[(size_t)j * p->m + m] = pre * p->wr[j] * p->wsigma[m] / p->sphere_area;
+    }
+  int rc = upload(&p->d_sigma, p->sigma.data(), p->sigma.size());
+  if (!rc) rc = upload(&p->d_rj, p->r.data(), p->r.size());
+  if (!rc) rc = upload(&p->d_wr, p->wr.data(), p->wr.size());
+  if (!rc) rc = upload(&p->d_wjm, wjm.data(), wjm.size());
+  if (!rc) rc = set_params(p, d->n_param, d->e_r, nullptr, p->d_wr);
+  if (!rc) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      set_error(std::string("table build failed: ") + hipGetErrorString(e));
+      rc = IB_ERR_HIP;
+    }
+  }
+  if (rc) {
+    ib_plan_destroy(p);
+    return rc;
+  }
+  *out = p;
+  return IB_OK;
+}
+
+int ib_plan_set_params(ib_plan* p, int n_param, const double* e_r, void* stream) {
+  if (!p || !e_r || n_param < 1) {
+    set_error("invalid parameter sets");
+    return IB_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  IB_HIP(hipStreamSynchronize(st));  // old tables may still be in use
+  int rc = set_params(p, n_param, e_r, st, p->d_wr);
+  if (rc) return rc;
+  IB_HIP(hipStreamSynchronize(st));
+  return IB_OK;
+}
+
+int ib_plan_destroy(ib_plan* p) {
+  if (!p) return IB_OK;
+  (void)hipDeviceSynchronize();
+  free_param_tables(p);
+  if (p->d_lam) (void)hipFree(p->d_lam);
+  if (p->d_sigma) (void)hipFree(p->d_sigma);
+  if (p->d_rj) (void)hipFree(p->d_rj);
+  if (p->d_wr) (void)hipFree(p->d_wr);
+  if (p->d_wjm) (void)hipFree(p->d_wjm);
+  p->fhat.release();
+  p->acc.release();
+  p->work.release();
+  p->aux.release();
+  delete p;
+  return IB_OK;
+}
+
+int ib_collide(ib_plan* p, const double* f, double* q, long long batch, const int* param_idx,
+               int with_loss, double heat_bath, void* stream) {
+  if (!p || (!f && batch > 0) || (!q && batch > 0) || batch < 0) {
+    set_error("invalid collide arguments");
+    return IB_ERR_ARG;
+  }
+  if (batch == 0) return IB_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (p->precision == IB_PREC_FP32) {
+    int handled = 0;
+    int rc = collide_fused(p, f, q, batch, param_idx, with_loss, heat_bath, st, &handled);
+    if (rc || handled) return rc;
+  }
+  return collide_plain(p, f, q, batch, param_idx, with_loss, heat_bath, 0, st);
+}
+
+int ib_laplacian(ib_plan* p, const double* f, double* out, long long batch, void* stream) {
+  if (!p || batch < 0 || (batch > 0 && (!f || !out))) {
+    set_error("invalid laplacian arguments");
+    return IB_ERR_ARG;
+  }
+  if (batch == 0) return IB_OK;
+  return collide_plain(p, f, out, batch, nullptr, 0, 1.0, 1, (hipStream_t)stream);
+}
+
+int ib_moments(const double* f, double* out, long long batch, int dim, int n, double L,
+               void* stream) {
+  if (batch < 0 || (dim != 2 && dim != 3) || n < 1 || (batch > 0 && (!f || !out))) {
+    set_error("invalid moments arguments");
+    return IB_ERR_ARG;
+  }
+  if (batch == 0) return IB_OK;
+  return launch_moments(f, out, batch, dim, n, L, (hipStream_t)stream);
+}
+
+int ib_project_conservative(double* q, long long batch, int dim, int n, double L, void* stream) {
+  if (batch < 0 || (dim != 2 && dim != 3) || n < 1 || (batch > 0 && !q)) {
+    set_error("invalid projection arguments");
+    return IB_ERR_ARG;
+  }
+  if (batch == 0) return IB_OK;
+  return launch_project(q, batch, dim, n, L, (hipStream_t)stream);
+}
+
+int ib_tvd1d(const double* f, double* df, int nx, long long nv, const double* v1, double dtdx,
+             int order, int limiter, int bc_lower, int bc_upper, int vx_rows, void* stream) {
+  if (bc_lower == 0 || bc_upper == 0) {
+    set_error(bc_lower == 0 ? "Lower boundary condition not specified."
+                            : "Upper boundary condition not specified.");
+    return IB_ERR_BC;
+  }
+  if ((bc_lower != 1 && bc_lower != 2) || (bc_upper != 1 && bc_upper != 2)) {
+    set_error("Boundary condition not implemented");
+    return IB_ERR_UNSUPPORTED;
+  }
+  if (nx < 1 || nv < 1 || !f || !df || !v1 || (order != 1 && order != 2) ||
+      (limiter != 0 && limiter != 1)) {
+    set_error("invalid tvd1d arguments");
+    return IB_ERR_ARG;
+  }
+  if ((bc_lower == 2 || bc_upper == 2) && nx < 2) {
+    set_error("periodic boundaries need nx >= 2");
+    return IB_ERR_ARG;
+  }
+  if (order == 2 && nx + 2 < vx_rows) {
+    // the reference slices s[LL-1:UL-1] along v_x and fails to broadcast
+    set_error("operands could not be broadcast together (nx + 2 < N)");
+    return IB_ERR_ARG;
+  }
+  return launch_tvd1d(f, df, nx, nv, v1, dtdx, order, limiter, bc_lower, bc_upper,
+                      (hipStream_t)stream);
+}
+
+int ib_axpy(double* y, const double* x, double a, long long n, void* stream) {
+  if (n < 0 || (n > 0 && (!y || !x))) {
+    set_error("invalid axpy arguments");
+    return IB_ERR_ARG;
+  }
+  if (n == 0) return IB_OK;
+  return launch_axpy(y, x, a, n, (hipStream_t)stream);
+}
+
+int ib_df_combine(const double* adv, const double* q, double* out, double dt, double kn,
+                  long long n, void* stream) {
+  if (n < 0 || (n > 0 && (!q || !out))) {
+    set_error("invalid df_combine arguments");
+    return IB_ERR_ARG;
+  }
+  if (n == 0) return IB_OK;
+  return launch_df_combine(adv, q, out, dt, kn, n, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,562 @@
+// Plain (staged) form of the fast spectral collision operator Q(f, f).
+//
+// Follows reference src/collision/inelastic_collisions.py:208-227 (collide):
+// C2C transforms over the d velocity axes, gain = ifft(sum_{j,m} G_jm *
+// fft(ifft(E_jm * fhat) * f)), loss = f * ifft(Lambda * fhat), real part at
+// the end.  The sum over j of the reference's Nr loss transforms is folded
+// into one transform of the combined symbol Lambda = sum_j l_j S_j (linear).
+//
+// Each N^d transform is split into an axis-0 pass over "pencils" (one thread
+// owns the N values along axis 0 of one pencil, in registers) and a pass over
+// the remaining axes ("slabs": a row in 2D, an N x N tile in LDS in 3D).  The
+// work array of every (cell, quadrature pair) item goes through HBM between
+// the passes; items are processed in chunks sized to a scratch budget.  The
+// element-wise products (E * fhat, * f, * G and the quadrature sum) are fused
+// into the passes and the E / G phase factors are generated per pair from 1D
+// tables (E and G are separable in k), so no (Nr, M, n) table is stored.
+//
+// Precision T = double gives the plain FP64 form; T = float the plain FP32 form.
+#include <algorithm>
+#include <type_traits>
+
+#include "common.h"
+#include "plan.h"
+
+namespace ib {
+
+enum { A_IN_PAIR = 0, A_OUT_PAIR = 1, A_OUT_PRO = 2, A_IN_EPI = 3 };
+enum { B_PAIR = 0, B_PRO = 1, B_EPI = 2 };
+
+template <int N, int D>
+struct Geo {
+  static constexpr int NP = (D == 2) ? N : N * N;  // pencils along axis 0 (= slab size)
+  static constexpr long long NPTS = (long long)N * NP;
+  static constexpr int NPW = NP < 64 ? NP : 64;  // pencils handled by one sub-group
+  static constexpr int SUBS = 64 / NPW;          // items per 64-thread workgroup
+  static constexpr int BLK = NP / NPW;           // pencil blocks per item
+};
+
+template <typename T>
+struct AArgs {
+  const cx<T>* src;
+  const cx<T>* src2;
+  cx<T>* dst;
+  long long items;
+  int P, p0, first;
+  const int* pidx;
+  long long cell0;
+  const double* sigma;
+  const double* rj;
+  const double* wjm;
+  const double* a;
+  const T* spg;
+  const T* lam;
+  int nr, m, nq, with_loss, no_gain;
+  double L, hb;
+};
+
+template <typename T>
+struct BArgs {
+  cx<T>* w;
+  const double* f;
+  double* q;
+  long long items;
+  int P, with_loss;
+};
+
+// ---------------------------------------------------------------------------
+// Axis-0 pass.  One thread per axis-0 pencil; 64-thread workgroups.
+template <typename T, int N, int D, int MODE>
+__global__ __launch_bounds__(64) void k_axis0(AArgs<T> A) {
+  using G = Geo<N, D>;
+  constexpr int NP = G::NP, NPW = G::NPW, SUBS = G::SUBS;
+  constexpr long long NPTS = G::NPTS;
+  __shared__ cx<T> tab[SUBS][D][N];
+  __shared__ cx<T> accs[MODE == A_OUT_PAIR ? N : 1][64];
+  const int tid = threadIdx.x;
+  const int sub = tid / NPW, lane = tid % NPW;
+  const long long item = (long long)blockIdx.y * SUBS + sub;
+  const int p = blockIdx.x * NPW + lane;
+  const bool active = item < A.items;
+  int k1 = 0, k2 = 0;
+  if (D == 2) {
+    k1 = p;
+  } else {
+    k1 = p / N;
+    k2 = p % N;
+  }
+  int qrest = kfreq(k1, N) * kfreq(k1, N);
+  if (D == 3) qrest += kfreq(k2, N) * kfreq(k2, N);
+  const T inv_n = (T)(1.0 / (double)NPTS);
+  cx<T> v[N];
+
+  if (MODE == A_IN_PAIR) {
+    const long long cell = item / A.P;
+    const int pair = A.p0 + (int)(item % A.P);
+    const int j = pair / A.m, mm = pair % A.m;
+    if (active) {
+      for (int t = lane; t < D * N; t += NPW) {
+        const int ax = t / N, k = t % N;
+        const double th = -(M_PI * A.rj[j] * A.sigma[mm * D + ax] * (double)kfreq(k, N)) / A.L;
+        double s, c;
+        sincos(th, &s, &c);
+        tab[sub][ax][k] = cx<T>{(T)c, (T)s};
+      }
+    }
+    __syncthreads();
+    if (!active) return;
+    cx<T> er = tab[sub][1][k1];
+    if (D == 3) er = er * tab[sub][2][k2];
+    er = inv_n * er;
+    const cx<T>* s = A.src + cell * NPTS + p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = s[(long long)i * NP] * (tab[sub][0][i] * er);
+    fft_reg<N, +1>(v);
+    cx<T>* d = A.dst + item * NPTS + p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[(long long)i * NP] = v[i];
+  } else if (MODE == A_OUT_PAIR) {
+    const long long cell = item;
+    const int ps = (active && A.pidx) ? A.pidx[A.cell0 + cell] : 0;
+    for (int pl = 0; pl < A.P; ++pl) {
+      const int pair = A.p0 + pl;
+      const int j = pair / A.m, mm = pair % A.m;
+      __syncthreads();
+      if (active) {
+        const double aj = A.a[ps * A.nr + j] * A.rj[j];
+        for (int t = lane; t < D * N; t += NPW) {
+          const int ax = t / N, k = t % N;
+          const double th = aj * A.sigma[mm * D + ax] * (double)kfreq(k, N);
+          double s, c;
+          sincos(th, &s, &c);
+          tab[sub][ax][k] = cx<T>{(T)c, (T)s};
+        }
+      }
+      __syncthreads();
+      if (!active) continue;
+      cx<T> gr = tab[sub][1][k1];
+      if (D == 3) gr = gr * tab[sub][2][k2];
+      const T wj = (T)A.wjm[j * A.m + mm];
+      const T* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
+      const cx<T>* s = A.src + (cell * A.P + pl) * NPTS + p;
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = s[(long long)i * NP];
+      fft_reg<N, -1>(v);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int ki = kfreq(i, N);
+        const T g = wj * sp[ki * ki + qrest];
+        const cx<T> t = (g * (tab[sub][0][i] * gr)) * v[i];
+        accs[i][tid] = (pl == 0) ? t : accs[i][tid] + t;
+      }
+    }
+    if (!active) return;
+    cx<T>* d = A.dst + cell * NPTS + p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const cx<T> t = accs[i][tid];
+      d[(long long)i * NP] = A.first ? t : d[(long long)i * NP] + t;
+    }
+  } else if (MODE == A_OUT_PRO) {
+    if (!active) return;
+    const cx<T>* s = A.src + item * NPTS + p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = s[(long long)i * NP];
+    fft_reg<N, -1>(v);
+    cx<T>* d = A.dst + item * NPTS + p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[(long long)i * NP] = v[i];
+  } else {  // A_IN_EPI
+    if (!active) return;
+    const long long cell = item;
+    const cx<T>* ac = A.src + cell * NPTS + p;
+    const cx<T>* fh = A.src2 + cell * NPTS + p;
+    const double lap_c = -(M_PI * M_PI) / (A.L * A.L);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      cx<T> x = A.no_gain ? cx<T>{(T)0, (T)0} : ac[(long long)i * NP];
+      if (A.hb != 0.0) {
+        const int ki = kfreq(i, N);
+        const T lk = (T)(A.hb * lap_c * (double)(ki * ki + qrest));
+        x = x + lk * fh[(long long)i * NP];
+      }
+      v[i] = inv_n * x;
+    }
+    fft_reg<N, +1>(v);
+    cx<T>* d = A.dst + (2 * cell) * NPTS + p;
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[(long long)i * NP] = v[i];
+    if (A.with_loss) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int ki = kfreq(i, N);
+        v[i] = (inv_n * A.lam[ki * ki + qrest]) * fh[(long long)i * NP];
+      }
+      fft_reg<N, +1>(v);
+      cx<T>* d2 = A.dst + (2 * cell + 1) * NPTS + p;
+#pragma unroll
+      for (int i = 0; i < N; ++i) d2[(long long)i * NP] = v[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Remaining-axes pass, 2D: one thread per row (axis 1 pencil, contiguous).
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(64) void k_rest2(BArgs<T> B) {
+  constexpr long long NPTS = (long long)N * N;
+  const long long gid = (long long)blockIdx.x * 64 + threadIdx.x;
+  const long long item = gid / N;
+  const int row = (int)(gid % N);
+  if (item >= B.items) return;
+  cx<T> v[N];
+  if (MODE == B_PAIR) {
+    const long long cell = item / B.P;
+    cx<T>* w = B.w + item * NPTS + (long long)row * N;
+    const double* f = B.f + cell * NPTS + (long long)row * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = w[i];
+    fft_reg<N, +1>(v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = (T)f[i] * v[i];
+    fft_reg<N, -1>(v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = v[i];
+  } else if (MODE == B_PRO) {
+    const double* f = B.f + item * NPTS + (long long)row * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = cx<T>{(T)f[i], (T)0};
+    fft_reg<N, -1>(v);
+    cx<T>* w = B.w + item * NPTS + (long long)row * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = v[i];
+  } else {  // B_EPI
+    const long long cell = item;
+    const double* f = B.f + cell * NPTS + (long long)row * N;
+    double* q = B.q + cell * NPTS + (long long)row * N;
+    T lr[N];
+    if (B.with_loss) {
+      const cx<T>* w2 = B.w + (2 * cell + 1) * NPTS + (long long)row * N;
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = w2[i];
+      fft_reg<N, +1>(v);
+#pragma unroll
+      for (int i = 0; i < N; ++i) lr[i] = v[i].re;
+    }
+    const cx<T>* w1 = B.w + (2 * cell) * NPTS + (long long)row * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = w1[i];
+    fft_reg<N, +1>(v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double out = (double)v[i].re;
+      if (B.with_loss) out = out - f[i] * (double)lr[i];
+      q[i] = out;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Remaining-axes pass, 3D: one N x N slab (fixed axis-0 index) per sub-group of
+// N threads, staged in LDS with rows padded to N + 1 (conflict-free columns
+// and rows).
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(64) void k_rest3(BArgs<T> B) {
+  constexpr int NP = N * N;
+  constexpr long long NPTS = (long long)N * NP;
+  constexpr int SPW = 64 / N;
+  __shared__ cx<T> tile[SPW][N][N + 1];
+  const int tid = threadIdx.x;
+  const int s = tid / N, l = tid % N;
+  const long long gs = (long long)blockIdx.x * SPW + s;
+  const long long item = gs / N;
+  const int i0 = (int)(gs % N);
+  const bool active = item < B.items;
+  cx<T> v[N];
+  auto col_pass = [&](auto dir_tag) {
+    constexpr int DIR = decltype(dir_tag)::value;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = tile[s][i][l];
+    fft_reg<N, DIR>(v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) tile[s][i][l] = v[i];
+  };
+  using FWD = std::integral_constant<int, -1>;
+  using INV = std::integral_constant<int, +1>;
+
+  if (MODE == B_PAIR) {
+    const long long cell = item / B.P;
+    cx<T>* w = B.w + item * NPTS + (long long)i0 * NP;
+    if (active) {
+#pragma unroll 4
+      for (int r = 0; r < N; ++r) tile[s][r][l] = w[r * N + l];
+    }
+    __syncthreads();
+    if (active) col_pass(INV{});
+    __syncthreads();
+    if (active) {
+      const double* f = B.f + cell * NPTS + (long long)i0 * NP + (long long)l * N;
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = tile[s][l][i];
+      fft_reg<N, +1>(v);
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = (T)f[i] * v[i];
+      fft_reg<N, -1>(v);
+#pragma unroll
+      for (int i = 0; i < N; ++i) tile[s][l][i] = v[i];
+    }
+    __syncthreads();
+    if (active) col_pass(FWD{});
+    __syncthreads();
+    if (active) {
+#pragma unroll 4
+      for (int r = 0; r < N; ++r) w[r * N + l] = tile[s][r][l];
+    }
+  } else if (MODE == B_PRO) {
+    if (active) {
+      const double* f = B.f + item * NPTS + (long long)i0 * NP + (long long)l * N;
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = cx<T>{(T)f[i], (T)0};
+      fft_reg<N, -1>(v);
+#pragma unroll
+      for (int i = 0; i < N; ++i) tile[s][l][i] = v[i];
+    }
+    __syncthreads();
+    if (active) col_pass(FWD{});
+    __syncthreads();
+    if (active) {
+      cx<T>* w = B.w + item * NPTS + (long long)i0 * NP;
+#pragma unroll 4
+      for (int r = 0; r < N; ++r) w[r * N + l] = tile[s][r][l];
+    }
+  } else {  // B_EPI
+    const long long cell = item;
+    T lr[N];
+    if (B.with_loss) {
+      if (active) {
+        const cx<T>* w2 = B.w + (2 * cell + 1) * NPTS + (long long)i0 * NP;
+#pragma unroll 4
+        for (int r = 0; r < N; ++r) tile[s][r][l] = w2[r * N + l];
+      }
+      __syncthreads();
+      if (active) col_pass(INV{});
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) v[i] = tile[s][l][i];
+        fft_reg<N, +1>(v);
+#pragma unroll
+        for (int i = 0; i < N; ++i) lr[i] = v[i].re;
+      }
+      __syncthreads();
+    }
+    if (active) {
+      const cx<T>* w1 = B.w + (2 * cell) * NPTS + (long long)i0 * NP;
+#pragma unroll 4
+      for (int r = 0; r < N; ++r) tile[s][r][l] = w1[r * N + l];
+    }
+    __syncthreads();
+    if (active) col_pass(INV{});
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = tile[s][l][i];
+      fft_reg<N, +1>(v);
+      const long long off = cell * NPTS + (long long)i0 * NP + (long long)l * N;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        double out = (double)v[i].re;
+        if (B.with_loss) out = out - B.f[off + i] * (double)lr[i];
+        B.q[off + i] = out;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, int N, int D, int MODE>
+static int launch_a(const AArgs<T>& A, hipStream_t st) {
+  using G = Geo<N, D>;
+  dim3 grid(G::BLK, (unsigned)((A.items + G::SUBS - 1) / G::SUBS));
+  hipLaunchKernelGGL((k_axis0<T, N, D, MODE>), grid, dim3(64), 0, st, A);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+template <typename T, int N, int D, int MODE>
+static int launch_b(const BArgs<T>& B, hipStream_t st) {
+  if (D == 2) {
+    const long long threads = B.items * N;
+    hipLaunchKernelGGL((k_rest2<T, N, MODE>), dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st,
+                       B);
+  } else {
+    constexpr int SPW = 64 / N;
+    const long long slabs = B.items * N;
+    hipLaunchKernelGGL((k_rest3<T, N, MODE>), dim3((unsigned)((slabs + SPW - 1) / SPW)), dim3(64), 0,
+                       st, B);
+  }
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+#define IB_TRY(x)               \
+  do {                          \
+    int _r = (x);               \
+    if (_r != IB_OK) return _r; \
+  } while (0)
+
+template <typename T, int N, int D>
+static int collide_staged(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                          int with_loss, double hb, int no_gain, hipStream_t st) {
+  using G = Geo<N, D>;
+  constexpr long long NPTS = G::NPTS;
+  const size_t arr = (size_t)NPTS * sizeof(cx<T>);
+  const int npairs = P->nr * P->m;
+  // sub-batch of cells and pairs per chunk, bounded by the scratch budget
+  long long bs = batch;
+  const long long max_cells = (long long)std::max<size_t>(1, P->scratch_budget / (2 * arr));
+  if (bs > max_cells) bs = max_cells;
+  long long pc = P->chunk_pairs > 0 ? P->chunk_pairs
+                                    : (long long)std::max<size_t>(1, P->scratch_budget / (bs * arr));
+  if (pc > npairs) pc = npairs;
+  if (pc < 2 && npairs >= 2) pc = 2;  // the epilogue needs 2 slots per cell
+  IB_TRY(P->fhat.ensure(bs * arr));
+  IB_TRY(P->acc.ensure(bs * arr));
+  IB_TRY(P->work.ensure((size_t)bs * (size_t)std::max<long long>(pc, 2) * arr));
+  cx<T>* fh = (cx<T>*)P->fhat.ptr;
+  cx<T>* ac = (cx<T>*)P->acc.ptr;
+  cx<T>* wk = (cx<T>*)P->work.ptr;
+
+  for (long long c0 = 0; c0 < batch; c0 += bs) {
+    const long long nb = std::min(bs, batch - c0);
+    const double* fc = f + c0 * NPTS;
+    double* qc = q + c0 * NPTS;
+    AArgs<T> A{};
+    A.sigma = P->d_sigma;
+    A.rj = P->d_rj;
+    A.wjm = P->d_wjm;
+    A.a = P->d_a;
+    A.spg = (const T*)P->d_spg;
+    A.lam = (const T*)P->d_lam;
+    A.nr = P->nr;
+    A.m = P->m;
+    A.nq = P->nq;
+    A.L = P->L;
+    A.hb = hb;
+    A.with_loss = with_loss;
+    A.no_gain = no_gain;
+    A.pidx = pidx;
+    A.cell0 = c0;
+    BArgs<T> B{};
+    B.f = fc;
+    B.q = qc;
+    B.with_loss = with_loss;
+    // prologue: fhat = fft(f)
+    B.w = wk;
+    B.items = nb;
+    B.P = 1;
+    IB_TRY((launch_b<T, N, D, B_PRO>(B, st)));
+    A.src = wk;
+    A.dst = fh;
+    A.items = nb;
+    IB_TRY((launch_a<T, N, D, A_OUT_PRO>(A, st)));
+    // quadrature pairs in chunks
+    for (int p0 = 0; p0 < (no_gain ? 0 : npairs); p0 += (int)pc) {
+      const int np = (int)std::min<long long>(pc, npairs - p0);
+      A.src = fh;
+      A.dst = wk;
+      A.items = nb * np;
+      A.P = np;
+      A.p0 = p0;
+      IB_TRY((launch_a<T, N, D, A_IN_PAIR>(A, st)));
+      B.w = wk;
+      B.items = nb * np;
+      B.P = np;
+      IB_TRY((launch_b<T, N, D, B_PAIR>(B, st)));
+      A.src = wk;
+      A.dst = ac;
+      A.items = nb;
+      A.first = (p0 == 0);
+      IB_TRY((launch_a<T, N, D, A_OUT_PAIR>(A, st)));
+    }
+    // epilogue: q = Re ifft(acc + hb lap fhat) - f Re ifft(Lambda fhat)
+    A.src = ac;
+    A.src2 = fh;
+    A.dst = wk;
+    A.items = nb;
+    IB_TRY((launch_a<T, N, D, A_IN_EPI>(A, st)));
+    B.w = wk;
+    B.items = nb;
+    B.P = 1;
+    IB_TRY((launch_b<T, N, D, B_EPI>(B, st)));
+  }
+  return IB_OK;
+}
+
+template <typename T>
+static int collide_staged_dispatch(Plan* P, const double* f, double* q, long long batch,
+                                   const int* pidx, int with_loss, double hb, int no_gain,
+                                   hipStream_t st) {
+  const int key = P->dim * 100 + P->n;
+  switch (key) {
+    case 208: return collide_staged<T, 8, 2>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
+    case 216: return collide_staged<T, 16, 2>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
+    case 232: return collide_staged<T, 32, 2>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
+    case 308: return collide_staged<T, 8, 3>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
+    case 316: return collide_staged<T, 16, 3>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
+    case 332: return collide_staged<T, 32, 3>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
+    default:
+      set_error("unsupported (dim, n) for the staged engine");
+      return IB_ERR_UNSUPPORTED;
+  }
+}
+
+int collide_plain(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                  int with_loss, double hb, int no_gain, hipStream_t st) {
+  if (P->precision == IB_PREC_FP64)
+    return collide_staged_dispatch<double>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
+  return collide_staged_dispatch<float>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
+}
+
+// ---------------------------------------------------------------------------
+// Table construction: sp(a_j r_j |k|) per (parameter set, j, |k|^2) and the
+// combined loss symbol Lambda(|k|^2) = sum_j |S| r_j^(gamma+d-1) w_j sp(pi r_j |k| / L).
+template <typename T>
+__global__ void k_tables(T* spg, T* lam, const double* a, const double* rj, const double* wr,
+                         int n_param, int nr, int nq, int dim, double L, double gamma, double S) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ng = (long long)n_param * nr * nq;
+  if (gid < ng) {
+    const int q = (int)(gid % nq);
+    const long long pj = gid / nq;
+    const int j = (int)(pj % nr);
+    const double x = (a[pj] * rj[j]) * sqrt((double)q);
+    spg[gid] = (T)sp_func(dim, x);
+  } else if (gid < ng + nq) {
+    const int q = (int)(gid - ng);
+    const double kn = sqrt((double)q);
+    double s = 0.0;
+    for (int j = 0; j < nr; ++j) {
+      const double lk = S * pow(rj[j], gamma + dim - 1) * wr[j];
+      s += lk * sp_func(dim, (M_PI * rj[j]) * kn / L);
+    }
+    lam[q] = (T)s;
+  }
+}
+
+int launch_tables(Plan* P, const double* d_wr, hipStream_t st) {
+  const long long total = (long long)P->n_param * P->nr * P->nq + P->nq;
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  if (P->precision == IB_PREC_FP64)
+    hipLaunchKernelGGL(k_tables<double>, dim3(blocks), dim3(256), 0, st, (double*)P->d_spg,
+                       (double*)P->d_lam, P->d_a, P->d_rj, d_wr, P->n_param, P->nr, P->nq, P->dim,
+                       P->L, P->gamma, P->sphere_area);
+  else
+    hipLaunchKernelGGL(k_tables<float>, dim3(blocks), dim3(256), 0, st, (float*)P->d_spg,
+                       (float*)P->d_lam, P->d_a, P->d_rj, d_wr, P->n_param, P->nr, P->nq, P->dim,
+                       P->L, P->gamma, P->sphere_area);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+}  // namespace ib

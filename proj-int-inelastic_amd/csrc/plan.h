@@ -1,0 +1,56 @@
+// Host-side plan of the spectral collision operator: quadrature, per-parameter
+// set tables, device scratch.  Shared by the plain (staged) engine and the
+// fused FP32 engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/inelastic.h"
+
+namespace ib {
+
+void set_error(const std::string& msg);
+
+#define IB_HIP(call)                                                                \
+  do {                                                                              \
+    hipError_t _e = (call);                                                         \
+    if (_e != hipSuccess) {                                                         \
+      ::ib::set_error(std::string(#call " failed: ") + hipGetErrorString(_e));      \
+      return IB_ERR_HIP;                                                            \
+    }                                                                               \
+  } while (0)
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want);  // grow-only; returns IB_OK or IB_ERR_HIP
+  void release();
+};
+
+struct Plan {
+  // geometry and quadrature (host copies)
+  int dim = 0, n = 0, nr = 0, m = 0, precision = 0;
+  double L = 0, gamma = 0;
+  long long npts = 0;  // n^dim
+  int nq = 0;          // number of |k|^2 values: dim (n/2)^2 + 1
+  int n_param = 0;
+  std::vector<double> r, wr, sigma, wsigma, e_r;
+  double sphere_area = 0;
+  // device tables
+  double* d_sigma = nullptr;  // [m][dim]
+  double* d_rj = nullptr;     // [nr]
+  double* d_wjm = nullptr;    // [nr][m]  r^(gamma+d-1) w_r w_sigma
+  double* d_a = nullptr;      // [n_param][nr]  pi (1 + e_j) / (4 L)
+  void* d_spg = nullptr;      // [n_param][nr][nq]  sp(a_j r_j |k|), T
+  void* d_lam = nullptr;      // [nq]  sum_j |S| r^(g+d-1) w_j sp(pi r_j |k| / L), T
+  // device scratch (grow-only)
+  DevBuf fhat, acc, work, aux;
+  long long chunk_pairs = 0;  // pairs per staged chunk (0 = automatic)
+  size_t scratch_budget = (size_t)1 << 30;
+};
+
+int build_tables(Plan* p, hipStream_t s);
+
+}  // namespace ib

@@ -1,0 +1,194 @@
+"""GPU parity of Q(f,f) through the C ABI against the FP64 oracle.
+
+Bars (relative L-infinity against the oracle's max |Q|):
+  plain FP64 form   <= 1e-12
+  FP32 forms        <= 1e-5
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import collision as OC
+from oracle import quadrature as OQ
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 1e-5
+
+
+def _cfg(d, N, nr, M, s, e, e0, gamma):
+    cfg = {"collision-model": {"dim": d, "e": e, "e0": e0, "gamma": gamma},
+           "velocity-mesh": {"nv": N, "s": s, "nr": nr}}
+    if d == 2:
+        cfg["circle-quad-rule"] = {"nphi": M}
+    else:
+        cfg["spherical-design-rule"] = {"ssrule": "lebedev", "nsphr": M}
+    return cfg
+
+
+def _oracle(d, N, nr, M, s, e, e0, gamma):
+    if d == 2:
+        sig, ws = OQ.circle_rule(M)
+    else:
+        sig, ws = OQ.lebedev_rule(M)
+    return OC.OracleOperator(d, N, s, nr, sig, ws, e=e, e0=e0, gamma=gamma)
+
+
+def _data(d, N, L, nb, seed):
+    rng = np.random.default_rng(seed)
+    dv = 2 * L / N
+    v = -L + (np.arange(N) + 0.5) * dv
+    grids = np.meshgrid(*([v] * d), indexing="ij")
+    out = []
+    for _ in range(nb):
+        T = rng.uniform(0.5, 2.0, d)
+        u = rng.normal(0, 0.3, d)
+        g = np.exp(-sum((grids[a] - u[a]) ** 2 / (2 * T[a]) for a in range(d)))
+        g2 = np.exp(-sum((grids[a] + u[a] + (0.7 if a == 0 else 0)) ** 2 / (2 * 0.6) for a in range(d)))
+        out.append(g + 0.5 * g2 + 1e-3 * rng.random(g.shape))
+    return np.stack(out)
+
+
+def _err(q, ref):
+    return float(np.abs(q - ref).max() / np.abs(ref).max())
+
+
+CASES = [
+    # d, N, nr, M, s, e, e0, gamma
+    (2, 8, 4, 4, 2.0, 0.7, 0.9, 1.0),
+    (2, 16, 8, 8, 2.0, 0.8, 0.8, 0.0),
+    (2, 32, 32, 8, 2.718491035932, 0.8, 0.8, 0.0),  # homogeneous cooling shape
+    (2, 32, 32, 8, 3.624654714576, 0.9, 0.9, 1.0),  # 1D x 2D shape (hard spheres)
+    (3, 8, 4, 6, 2.0, 0.6, 0.95, 1.0),
+    (3, 16, 8, 14, 3.0, 0.5, 0.5, 1.0),
+    (3, 32, 6, 14, 3.624654714576, 0.75, 0.75, 1.0),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "d%dN%dnr%dM%d" % c[:4])
+@pytest.mark.parametrize("with_loss", [False, True])
+def test_fp64_matches_oracle(pkg, need_gpu, case, with_loss):
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    d, N, nr, M, s, e, e0, g = case
+    cfg = _cfg(d, N, nr, M, s, e, e0, g)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm)
+    op = _oracle(d, N, nr, M, s, e, e0, g)
+    f = _data(d, N, sm.L, 2, seed=N + d)
+    ref = op.collide(f, with_loss=with_loss)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=with_loss).cpu().numpy()
+    err = _err(q, ref)
+    print("case", case, "with_loss", with_loss, "FP64 rel Linf", err)
+    assert err <= TOL64, err
+
+
+@pytest.mark.parametrize("case", CASES[1:], ids=lambda c: "d%dN%dnr%dM%d" % c[:4])
+def test_fp32_within_tolerance(pkg, need_gpu, case):
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    d, N, nr, M, s, e, e0, g = case
+    cfg = _cfg(d, N, nr, M, s, e, e0, g)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision="fp32")
+    op = _oracle(d, N, nr, M, s, e, e0, g)
+    f = _data(d, N, sm.L, 2, seed=7)
+    ref = op.collide(f, with_loss=True)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True).cpu().numpy()
+    err = _err(q, ref)
+    print("case", case, "FP32 rel Linf", err)
+    assert err <= TOL32, err
+
+
+def test_full_3d_32_single_cell_fp64(pkg, need_gpu):
+    """Full velocity-space shape of the 3D batch: 32^3, Nr = 32, M = 38 Lebedev."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    case = (3, 32, 32, 38, 3.624654714576, 0.65, 0.65, 1.0)
+    cfg = _cfg(*case)
+    sm = SpectralMesh(cfg)
+    op = _oracle(*case)
+    f = _data(3, 32, sm.L, 1, seed=11)
+    ref = op.collide(f, with_loss=True, workers=8)
+    for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
+        coll = FSInelasticVHSCollision(cfg, sm, precision=prec)
+        q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True).cpu().numpy()
+        err = _err(q, ref)
+        print("3D 32^3 Nr32 M38", prec, "rel Linf", err)
+        assert err <= tol, (prec, err)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", TOL64), ("fp32", TOL32)])
+def test_per_cell_restitution(pkg, need_gpu, prec, tol):
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    d, N, nr, M, s = 3, 16, 8, 14, 3.0
+    cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 1.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision=prec)
+    es = np.array([0.5, 0.9, 0.5, 0.73])
+    f = _data(d, N, sm.L, 4, seed=3)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True, e=es).cpu().numpy()
+    for c in range(4):
+        ref = _oracle(d, N, nr, M, s, es[c], es[c], 1.0).collide(f[c], with_loss=True)
+        assert _err(q[c], ref) <= tol, (c, _err(q[c], ref))
+    # back to the configured restitution
+    q0 = coll.collide(torch.from_numpy(f[:1]).cuda(), with_loss=True).cpu().numpy()
+    ref0 = _oracle(d, N, nr, M, s, 1.0, 1.0, 1.0).collide(f[:1], with_loss=True)
+    assert _err(q0, ref0) <= tol
+
+
+def test_heat_bath_laplacian_and_call(pkg, need_gpu):
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    case = (2, 16, 8, 8, 2.0, 0.8, 1.0, 0.0)
+    cfg = _cfg(*case)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm)
+    op = _oracle(*case)
+    f = _data(2, 16, sm.L, 3, seed=5)
+    # numpy in -> numpy out, gain only + heat bath, as base.py:32-58
+    q = coll(f, heat_bath=0.25, device="cpu")
+    assert isinstance(q, np.ndarray)
+    ref = op(f, heat_bath=0.25)
+    assert _err(q, ref) <= TOL64
+    lap = coll.laplacian(f)
+    assert _err(lap, op.laplacian(f)) <= TOL64
+
+
+def test_projection_and_moments(pkg, need_gpu):
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    case = (3, 16, 8, 14, 3.0, 0.6, 0.6, 1.0)
+    cfg = _cfg(*case)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm)
+    op = _oracle(*case)
+    f = _data(3, 16, sm.L, 3, seed=9)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True, project=True).cpu().numpy()
+    ref = OC.project_conservative(op.collide(f, with_loss=True), op.v_grids())
+    assert _err(q, ref) <= TOL64
+    mom = coll.moments(q)
+    scale = np.abs(q).sum(axis=(1, 2, 3)) * sm.delta ** 3
+    assert np.all(np.abs(mom[:, 0]) <= 1e-13 * scale)
+    assert np.all(np.abs(mom[:, 1:4]) <= 1e-13 * scale[:, None] * sm.L)
+    mf = coll.moments(f)
+    r, m, en = OC.moments(f, op.v_grids(), op.delta)
+    assert np.allclose(mf[:, 0], r, rtol=1e-14)
+    assert np.allclose(mf[:, 1:4], m, rtol=1e-13, atol=1e-14 * np.abs(r).max())
+    assert np.allclose(mf[:, 4], en, rtol=1e-14)
+
+
+def test_batch_invariance_fp64(pkg, need_gpu):
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    case = (2, 32, 16, 8, 3.0, 0.9, 0.9, 1.0)
+    cfg = _cfg(*case)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm)
+    f = _data(2, 32, sm.L, 5, seed=2)
+    qa = coll.collide(torch.from_numpy(f).cuda(), with_loss=True).cpu().numpy()
+    for c in range(5):
+        qc = coll.collide(torch.from_numpy(f[c]).cuda(), with_loss=True).cpu().numpy()
+        assert _err(qc, qa[c]) <= 1e-13
