@@ -1,0 +1,197 @@
+"""Benchmark: Q(f,f) cell-evaluations/s on the 3D 32^3 batch (one MI355X).
+
+Workload (the 3D velocity batch): 1024 independent cells, 32^3 velocity grid on
+[-8, 8]^3 (s = 3.624654714576), Nr = 32 Gauss-Legendre radial nodes, M = 38
+Lebedev angles, inelastic hard spheres (gamma = 1), per-cell restitution
+e ~ U[0.5, 1] (e0 = e), anisotropic Gaussian initial data (per-axis T ~
+U[0.5, 2], mean velocity ~ N(0, 0.3^2)), gain + loss, followed by the FP64
+conservative projection.  Synthetic data, seeded.
+
+One step = one Q evaluation of the whole batch (inputs resident in HBM).
+``value`` is the FP32 form's cell-evals/s; the plain FP64 form is timed on the
+same batch and reported beside it with the speed-up; the FP32 error is
+measured against the FP64 oracle on a sample of cells.  ``roofline`` is for
+the dominant kernel of the FP32 form (VALU-bound FFT work, algorithmic flops
+per cell-eval = (2 Nr M + Nr + 2) 5 n log2 n + (16 Nr M + 7 Nr) n).
+``cpu_baseline`` times the FP64 numpy/scipy oracle (8 FFT worker threads) on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import math
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+S3 = 3.624654714576
+N, NR, M, D = 32, 32, 38, 3
+
+
+def flops_per_cell(n=N ** D, nr=NR, m=M):
+    return (2 * nr * m + nr + 2) * 5 * n * math.log2(n) + (16 * nr * m + 7 * nr) * n
+
+
+def make_batch(nb, L, seed=0):
+    rng = np.random.default_rng(seed)
+    dv = 2 * L / N
+    v = -L + (np.arange(N) + 0.5) * dv
+    g = np.meshgrid(v, v, v, indexing="ij")
+    f = np.empty((nb, N, N, N))
+    for c in range(nb):
+        T = rng.uniform(0.5, 2.0, 3)
+        u = rng.normal(0.0, 0.3, 3)
+        x = np.exp(-sum((g[a] - u[a]) ** 2 / (2 * T[a]) for a in range(3)))
+        f[c] = x / (x.sum() * dv ** 3)
+    e = rng.uniform(0.5, 1.0, nb)
+    return f, e
+
+
+def cfg3(prec_e=1.0):
+    return {"collision-model": {"dim": 3, "e": prec_e, "e0": prec_e, "gamma": 1.0},
+            "velocity-mesh": {"nv": N, "s": S3, "nr": NR},
+            "spherical-design-rule": {"ssrule": "lebedev", "nsphr": M}}
+
+
+def _load_pkg():
+    import importlib.util
+    name = "proj_int_inelastic_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    pkg = ROOT / "proj-int-inelastic_amd"
+    spec = importlib.util.spec_from_file_location(name, pkg / "__init__.py",
+                                                  submodule_search_locations=[str(pkg)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def time_path(coll, f_dev, e, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        coll.collide(f_dev, with_loss=True, e=e, project=True)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(st)
+    q = None
+    for _ in range(steps):
+        q = coll.collide(f_dev, with_loss=True, e=e, project=True)
+    ev1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return ev0.elapsed_time(ev1) / 1e3 / steps, wall / steps, q
+
+
+def cpu_baseline(nb_sample, seed):
+    """FP64 oracle (scipy.fft, 8 worker threads) on a bounded sample of cells."""
+    from oracle import collision as OC
+    from oracle import quadrature as OQ
+    sig, ws = OQ.lebedev_rule(M)
+    L = 0.5 * (3 + math.sqrt(2)) * S3
+    f, e = make_batch(nb_sample, L, seed)
+    workers = 8
+    ops = [OC.OracleOperator(3, N, S3, NR, sig, ws, e=e[c], e0=e[c], gamma=1.0)
+           for c in range(nb_sample)]  # precomputation excluded, as the reference's setup
+    t0 = time.perf_counter()
+    for c in range(nb_sample):
+        ops[c].collide(f[c], with_loss=True, workers=workers)
+    dt = time.perf_counter() - t0
+    return nb_sample / dt, workers, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=1024)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--check-cells", type=int, default=2)
+    ap.add_argument("--cpu-cells", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the Q(f,f) batch is sharded by cells with no "
+              "exchange, so a multi-GPU figure would be N x the one-GPU value; run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    pkg = _load_pkg()
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+
+    cfg = cfg3()
+    sm = SpectralMesh(cfg)
+    f, e = make_batch(args.cells, sm.L, seed=0)
+    f_dev = torch.from_numpy(f).cuda()
+    c32 = FSInelasticVHSCollision(cfg, sm, precision="fp32")
+    t32, w32, q32 = time_path(c32, f_dev, e, args.steps, args.warmup)
+    res = {"fp32_s_per_step": t32}
+    if not args.no_fp64:
+        c64 = FSInelasticVHSCollision(cfg, sm, precision="fp64")
+        t64, w64, q64 = time_path(c64, f_dev, e, args.fp64_steps, 1 if args.warmup else 0)
+        res["fp64_s_per_step"] = t64
+        d = (q32 - q64).abs().amax(dim=(1, 2, 3)) / q64.abs().amax(dim=(1, 2, 3))
+        res["fp32_vs_fp64_gpu_rel_linf_max"] = float(d.max())
+    # error of the FP32 form against the FP64 oracle on a sample of cells
+    from oracle import collision as OC
+    from oracle import quadrature as OQ
+    sig, ws = OQ.lebedev_rule(M)
+    errs = []
+    for c in range(min(args.check_cells, args.cells)):
+        op = OC.OracleOperator(3, N, S3, NR, sig, ws, e=e[c], e0=e[c], gamma=1.0)
+        ref = OC.project_conservative(op.collide(f[c], with_loss=True, workers=8), op.v_grids())
+        qc = q32[c].cpu().numpy()
+        errs.append(float(np.abs(qc - ref).max() / np.abs(ref).max()))
+    # conservation of the projected FP32 result (FP64 moments)
+    mom = c32.moments(q32)
+    fm = c32.moments(f_dev)
+    mass_drift = float((mom[:, 0].abs() / fm[:, 0].abs()).max())
+    cells_s = args.cells / t32
+    flop = flops_per_cell()
+    achieved = flop * cells_s / 1e12
+    peak = 157.3
+    out = {
+        "metric": "Q(f,f) cell-evals/s at N=32^3 (FP32 fast form, rel Linf vs FP64 oracle <= 1e-5)",
+        "value": cells_s,
+        "unit": "cell-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (complex FFTs, Kahan quadrature sums) + fp64 moments/projection",
+        "data": "synthetic (seeded anisotropic Gaussians, e ~ U[0.5,1])",
+        "config": {"workload": "3D velocity batch: 1024 cells x 32^3, Nr=32, M=38 Lebedev, "
+                               "hard spheres, per-cell e, gain+loss+projection",
+                   "cells": args.cells, "n": N, "nr": NR, "m": M},
+        "fp32_rel_linf_vs_oracle_max": max(errs) if errs else None,
+        "fp32_mass_rel_drift_max": mass_drift,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None},
+    }
+    if "fp64_s_per_step" in res:
+        out["fp64_plain_cell_evals_per_s"] = args.cells / res["fp64_s_per_step"]
+        out["fp32_over_fp64"] = res["fp64_s_per_step"] / t32
+        out["fp32_vs_fp64_gpu_rel_linf_max"] = res["fp32_vs_fp64_gpu_rel_linf_max"]
+    if not args.no_cpu:
+        rate, cores, dt = cpu_baseline(args.cpu_cells, seed=1)
+        out["cpu_baseline"] = {"value": rate, "unit": "cell-evals/s", "cores": cores, "kind": "port",
+                               "sample": "%d cell(s) of the same workload, FP64 oracle, %.1f s"
+                                         % (args.cpu_cells, dt)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -59,6 +59,13 @@ int ib_plan_set_params(ib_plan* plan, int n_param, const double* e_r, void* stre
 
 int ib_plan_destroy(ib_plan* plan);
 
+/* Tuning knobs of a plan (0 keeps the automatic choice): quadrature pairs per
+ * staged chunk, cells per fused launch, scratch budget in bytes of the staged
+ * engine.  New (the reference's FFTW plans have no such knobs); used by the
+ * autotuner. */
+int ib_plan_set_tuning(ib_plan* plan, long long chunk_pairs, long long fused_cells,
+                       long long scratch_budget);
+
 /* Q(f, f) for `batch` cells: f, q are (batch, n^dim) float64 device arrays.
  * param_idx: device int32 (batch,) parameter-set index per cell, or NULL for 0.
  * with_loss = 0 returns the gain term only (the reference default);

@@ -96,6 +96,12 @@ class FSInelasticVHSCollision(BaseCollision):
                 pass
             self._plan = None
 
+    def set_tuning(self, chunk_pairs=0, fused_cells=0, scratch_budget=0):
+        """Plan knobs (0 = automatic): pairs per staged chunk, cells per fused launch,
+        staged scratch budget in bytes."""
+        _lib.check(_lib.load().ib_plan_set_tuning(self._plan, int(chunk_pairs), int(fused_cells),
+                                                  int(scratch_budget)), "ib_plan_set_tuning")
+
     def set_param_sets(self, e_r):
         """Install restitution profiles (n_param, nr); cells select one by index."""
         e_r = np.ascontiguousarray(e_r, dtype=np.float64)

@@ -70,6 +70,7 @@ static int set_params(Plan* p, int n_param, const double* e_r, hipStream_t st, c
   const size_t tsz = p->precision == IB_PREC_FP64 ? sizeof(double) : sizeof(float);
   IB_HIP(hipMalloc(&p->d_spg, (size_t)n_param * p->nr * p->nq * tsz));
   if (!p->d_lam) IB_HIP(hipMalloc(&p->d_lam, (size_t)p->nq * tsz));
+  if (!p->d_lam64) IB_HIP(hipMalloc((void**)&p->d_lam64, (size_t)p->nq * sizeof(double)));
   rc = launch_tables(p, d_wr, st);
   if (rc) return rc;
   return IB_OK;
@@ -174,11 +175,24 @@ int ib_plan_set_params(ib_plan* p, int n_param, const double* e_r, void* stream)
   return IB_OK;
 }
 
+int ib_plan_set_tuning(ib_plan* p, long long chunk_pairs, long long fused_cells,
+                       long long scratch_budget) {
+  if (!p || chunk_pairs < 0 || fused_cells < 0 || scratch_budget < 0) {
+    set_error("invalid tuning values");
+    return IB_ERR_ARG;
+  }
+  p->chunk_pairs = chunk_pairs;
+  p->fused_cells = fused_cells;
+  if (scratch_budget > 0) p->scratch_budget = (size_t)scratch_budget;
+  return IB_OK;
+}
+
 int ib_plan_destroy(ib_plan* p) {
   if (!p) return IB_OK;
   (void)hipDeviceSynchronize();
   free_param_tables(p);
   if (p->d_lam) (void)hipFree(p->d_lam);
+  if (p->d_lam64) (void)hipFree(p->d_lam64);
   if (p->d_sigma) (void)hipFree(p->d_sigma);
   if (p->d_rj) (void)hipFree(p->d_rj);
   if (p->d_wr) (void)hipFree(p->d_wr);
@@ -199,6 +213,7 @@ int ib_collide(ib_plan* p, const double* f, double* q, long long batch, const in
   }
   if (batch == 0) return IB_OK;
   hipStream_t st = (hipStream_t)stream;
+  p->stream = st;
   if (p->precision == IB_PREC_FP32) {
     int handled = 0;
     int rc = collide_fused(p, f, q, batch, param_idx, with_loss, heat_bath, st, &handled);

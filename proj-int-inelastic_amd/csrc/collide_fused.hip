@@ -1,12 +1,372 @@
-// Fused FP32 engine entry (filled in by the fused kernels).
+// Fused FP32 form of Q(f, f) for 3D 32^3 grids (the 3D velocity batch).
+//
+// Same operator as reference src/collision/inelastic_collisions.py:208-227,
+// computed with real-data transforms: h = f * ifft(E fhat) is real, so each
+// 32^3 transform needs only the kx in [0, 16] half spectrum; the kx = 0 and
+// kx = 16 planes (real after the x transform) are packed into one complex
+// plane.  The reference's C2C result is recovered up to O(|fhat_Nyquist|^2)
+// because its final real part only sees the Hermitian part of each product,
+// which is what the real-data form computes when the Nyquist entry of every
+// 1D phase factor is replaced by its real part (cos).
+//
+// Work decomposition: one 512-thread workgroup per (cell, radial node j)
+// loops over the M angles.  The 16 packed kx-planes (32 x 33 padded complex
+// FP32 = 132 KiB) stay in LDS for the whole pair; the pass along kz is done
+// with the pencil in registers and fused with E * fhat on the way in and with
+// G * (.) plus the Kahan-compensated quadrature sum on the way out; the
+// pass along x turns two real pencils into one complex FFT.  Workgroups of
+// one cell are numbered so they land on the same XCD (the dispatcher assigns
+// workgroup i to XCD i mod 8), so fhat and f of a cell are read from one L2.
+// Per-(cell, j) Kahan sums (hi, lo) go to a partial buffer; the FP64 reduce
+// over j, the final inverse transform, the loss term and the Laplacian run in
+// FP64 (collide_plain.hip kernels), as does the prologue fhat = fft(f).
+#include <algorithm>
+
+#include "common.h"
 #include "plan.h"
+
+// 1: Kahan-compensated FP32 sum over the M angles inside a workgroup (needs
+// 64 more registers per thread); 0: plain FP32 sum over the M angles.  The
+// sum over radial nodes is done in FP64 either way (k_f3_reduce).
+#ifndef IB_FUSED_KAHAN
+#define IB_FUSED_KAHAN 0
+#endif
 
 namespace ib {
 
+// FP64 staged-engine pieces (collide_plain.hip)
+int staged_prologue_fp64_332(Plan* P, const double* f, long long nb, void* fhat64, void* work);
+int staged_epilogue_fp64_332(Plan* P, const double* f, double* q, long long nb, void* acc64,
+                             void* fhat64, void* work, int with_loss, double hb, long long cell0);
+
+namespace f3 {
+constexpr int N = 32;
+constexpr int NPL = 16;  // packed kx planes
+constexpr int PAD = N + 1;
+constexpr int THREADS = 512;
+constexpr long long NPTS = (long long)N * N * N;
+constexpr long long PLANE = (long long)N * N;
+}  // namespace f3
+
+struct F3Args {
+  const cx<float>* fh;  // [cell][17][32][32], kx = 0..16, scaled by 1/n
+  const float* f32;     // [cell][32][32][32]
+  cx<float>* part;      // [cell][nr][2][16][32][32] (hi, lo)
+  const int* pidx;
+  long long batch;
+  long long cell0;
+  const double* sigma;
+  const double* rj;
+  const double* wjm;
+  const double* a;
+  const float* spg;
+  int nr, m, nq;
+  double L;
+};
+
+__device__ __forceinline__ cx<float> mulf(cx<float> a, cx<float> b) { return a * b; }
+
+#if IB_FUSED_KAHAN
+__device__ __forceinline__ void acc_add(cx<float>& s, cx<float>& c, cx<float> x) {
+  const float yr = x.re - c.re;
+  const float tr = s.re + yr;
+  c.re = (tr - s.re) - yr;
+  s.re = tr;
+  const float yi = x.im - c.im;
+  const float ti = s.im + yi;
+  c.im = (ti - s.im) - yi;
+  s.im = ti;
+}
+#else
+__device__ __forceinline__ void acc_add(cx<float>& s, cx<float>& c, cx<float> x) {
+  (void)c;
+  s = s + x;
+}
+#endif
+
+__global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
+  using namespace f3;
+  __shared__ cx<float> W[NPL][N][PAD];
+  __shared__ cx<float> tE[3][N];
+  __shared__ cx<float> tG[3][N];
+  __shared__ float sp[3 * (N / 2) * (N / 2) + 1];
+  const int tid = threadIdx.x;
+  const long long id = blockIdx.x;
+  const int c8 = (int)(id % 8);
+  const int j = (int)((id / 8) % A.nr);
+  const long long cg = id / (8LL * A.nr);
+  const long long cell = cg * 8 + c8;
+  if (cell >= A.batch) return;  // uniform over the workgroup
+  const int ps = A.pidx ? A.pidx[A.cell0 + cell] : 0;
+  for (int q = tid; q < A.nq; q += THREADS) sp[q] = A.spg[((long long)ps * A.nr + j) * A.nq + q];
+  const double aj = A.a[ps * A.nr + j] * A.rj[j];
+  const double rpl = M_PI * A.rj[j] / A.L;
+  const int p = tid / N, l = tid % N;                 // plane-pass role
+  const int xy = tid / 16, xz = 2 * (tid % 16);        // x-pass role: pencils (xy, xz), (xy, xz+1)
+  const cx<float>* fh = A.fh + (A.cell0 + cell) * 17 * PLANE;
+  const float* fc = A.f32 + (A.cell0 + cell) * NPTS;
+  const int kyf = kfreq(l, N);
+
+  cx<float> acc[N];
+#if IB_FUSED_KAHAN
+  cx<float> cmp[N];
+#else
+  cx<float> cmp[1];
+#endif
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = cx<float>{0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(cmp) / sizeof(cmp[0])); ++k) cmp[k] = cx<float>{0.f, 0.f};
+
+  for (int mm = 0; mm < A.m; ++mm) {
+    __syncthreads();
+    if (tid < 6 * N) {
+      const int which = tid / (3 * N), ax = (tid % (3 * N)) / N, k = tid % N;
+      const double s = A.sigma[mm * 3 + ax];
+      const double kf = (double)kfreq(k, N);
+      const double th = which == 0 ? -(rpl * s * kf) : (aj * s * kf);
+      double sn, cs;
+      sincos(th, &sn, &cs);
+      if (k == N / 2) sn = 0.0;  // Nyquist entry: real part only
+      (which ? tG : tE)[ax][k] = cx<float>{(float)cs, (float)sn};
+    }
+    const float wj = (float)A.wjm[j * A.m + mm];
+    __syncthreads();
+
+    cx<float> v[N];
+    // (1) E * fhat, inverse transform along kz (pencil (p, ky = l) in registers)
+    {
+      const cx<float> ey = tE[1][l];
+      if (p == 0) {
+        const cx<float> e0 = tE[0][0] * ey, e16 = tE[0][N / 2] * ey;
+        const cx<float>* f0 = fh + (long long)l * N;
+        const cx<float>* f16 = fh + 16 * PLANE + (long long)l * N;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const cx<float> a0 = (e0 * tE[2][k]) * f0[k];
+          const cx<float> a1 = (e16 * tE[2][k]) * f16[k];
+          v[k] = cx<float>{a0.re - a1.im, a0.im + a1.re};
+          if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        }
+      } else {
+        const cx<float> ex = tE[0][p] * ey;
+        const cx<float>* fp = fh + (long long)p * PLANE + (long long)l * N;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          v[k] = (ex * tE[2][k]) * fp[k];
+          if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      fft_reg<N, +1>(v);
+#pragma unroll
+      for (int k = 0; k < N; ++k) W[p][l][k] = v[k];
+    }
+    __syncthreads();
+    // (2) inverse along ky (pencil (p, z = l))
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = W[p][k][l];
+    fft_reg<N, +1>(v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) W[p][k][l] = v[k];
+    __syncthreads();
+    // (3) along x: two real pencils (xy, xz), (xy, xz + 1) as one complex FFT
+    {
+      cx<float> Z[N];
+      {
+        const cx<float> a0 = W[0][xy][xz], b0 = W[0][xy][xz + 1];
+        Z[0] = cx<float>{a0.re, b0.re};
+        Z[N / 2] = cx<float>{a0.im, b0.im};
+      }
+#pragma unroll
+      for (int k = 1; k < N / 2; ++k) {
+        const cx<float> a = W[k][xy][xz], b = W[k][xy][xz + 1];
+        Z[k] = cx<float>{a.re - b.im, a.im + b.re};
+        Z[N - k] = cx<float>{a.re + b.im, b.re - a.im};
+      }
+      fft_reg<N, +1>(Z);
+      // 32-bit per-thread offset, made opaque per pair so the 32 addresses are
+      // formed next to their loads instead of being hoisted out of the loop
+      int toff = xy * N + xz;
+      asm volatile("" : "+v"(toff));
+#pragma unroll
+      for (int x = 0; x < N; ++x) {
+        const float fa = fc[x * (int)PLANE + toff], fb = fc[x * (int)PLANE + toff + 1];
+        Z[x] = cx<float>{Z[x].re * fa, Z[x].im * fb};
+        if ((x & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+      fft_reg<N, -1>(Z);
+      W[0][xy][xz] = cx<float>{Z[0].re, Z[N / 2].re};
+      W[0][xy][xz + 1] = cx<float>{Z[0].im, Z[N / 2].im};
+#pragma unroll
+      for (int k = 1; k < N / 2; ++k) {
+        const cx<float> zk = Z[k], zm = Z[N - k];
+        W[k][xy][xz] = cx<float>{0.5f * (zk.re + zm.re), 0.5f * (zk.im - zm.im)};
+        W[k][xy][xz + 1] = cx<float>{0.5f * (zk.im + zm.im), 0.5f * (zm.re - zk.re)};
+      }
+    }
+    __syncthreads();
+    // (4) forward along ky
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = W[p][k][l];
+    fft_reg<N, -1>(v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) W[p][k][l] = v[k];
+    __syncthreads();
+    // (5) forward along kz, then G * (.) and the compensated quadrature sum
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = W[p][l][k];
+    fft_reg<N, -1>(v);
+    if (p == 0) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) W[0][l][k] = v[k];
+    }
+    __syncthreads();
+    {
+      const cx<float> gy = tG[1][l];
+      if (p == 0) {
+        const int lm = (N - l) % N;
+        const cx<float> g0 = tG[0][0] * gy, g16 = tG[0][N / 2] * gy;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const int kzf = kfreq(k, N);
+          const int qq = kyf * kyf + kzf * kzf;
+          const cx<float> pr = W[0][lm][(N - k) % N];
+          const cx<float> h0{0.5f * (v[k].re + pr.re), 0.5f * (v[k].im - pr.im)};
+          const cx<float> h16{0.5f * (v[k].im + pr.im), 0.5f * (pr.re - v[k].re)};
+          const cx<float> G0 = (wj * sp[qq]) * (g0 * tG[2][k]);
+          const cx<float> G16 = (wj * sp[qq + (N / 2) * (N / 2)]) * (g16 * tG[2][k]);
+          const cx<float> t0 = G0 * h0, t1 = G16 * h16;
+          acc_add(acc[k], cmp[IB_FUSED_KAHAN ? k : 0], cx<float>{t0.re - t1.im, t0.im + t1.re});
+        }
+      } else {
+        const cx<float> gxy = tG[0][p] * gy;
+        const int q0 = p * p + kyf * kyf;
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const int kzf = kfreq(k, N);
+          const cx<float> G = (wj * sp[q0 + kzf * kzf]) * (gxy * tG[2][k]);
+          acc_add(acc[k], cmp[IB_FUSED_KAHAN ? k : 0], G * v[k]);
+        }
+      }
+    }
+  }
+  cx<float>* out = A.part + ((A.cell0 + cell) * A.nr + j) * 2 * NPL * PLANE;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    out[(long long)p * PLANE + (long long)l * N + k] = acc[k];
+    const cx<float> c = cmp[IB_FUSED_KAHAN ? k : 0];
+    out[NPL * PLANE + (long long)p * PLANE + (long long)l * N + k] = cx<float>{-c.re, -c.im};
+  }
+}
+
+// fhat (FP64, full spectrum) -> kx = 0..16 planes in FP32, scaled by 1/n; f -> FP32.
+__global__ void k_f3_pack(const cx<double>* __restrict__ fhat, const double* __restrict__ f,
+                          cx<float>* __restrict__ fh32, float* __restrict__ f32, long long nb) {
+  using namespace f3;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long per = NPTS;
+  if (gid >= nb * per) return;
+  const long long c = gid / per, i = gid % per;
+  f32[gid] = (float)f[gid];
+  const int kx = (int)(i / PLANE);
+  if (kx <= N / 2) {
+    const cx<double> v = fhat[gid];
+    const double s = 1.0 / (double)NPTS;
+    fh32[c * 17 * PLANE + i] = cx<float>{(float)(v.re * s), (float)(v.im * s)};
+  }
+}
+
+// Sum the per-j Kahan partials in FP64 and expand the packed half spectrum to
+// the full FP64 spectrum (Hermitian completion), as the FP64 epilogue expects.
+__global__ void k_f3_reduce(const cx<float>* __restrict__ part, cx<double>* __restrict__ acc64,
+                            long long nb, int nr) {
+  using namespace f3;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nb * NPTS) return;
+  const long long c = gid / NPTS, i = gid % NPTS;
+  const int kx = (int)(i / PLANE), ky = (int)((i / N) % N), kz = (int)(i % N);
+  auto sum_at = [&](int pl, int yy, int zz) {
+    double re = 0.0, im = 0.0;
+    const cx<float>* b = part + c * nr * 2 * NPL * PLANE + (long long)pl * PLANE + (long long)yy * N + zz;
+    for (int j = 0; j < nr; ++j) {
+      const cx<float> hi = b[(long long)j * 2 * NPL * PLANE];
+      const cx<float> lo = b[(long long)j * 2 * NPL * PLANE + NPL * PLANE];
+      re += (double)hi.re + (double)lo.re;
+      im += (double)hi.im + (double)lo.im;
+    }
+    return cx<double>{re, im};
+  };
+  const int my = (N - ky) % N, mz = (N - kz) % N;
+  cx<double> out;
+  if (kx == 0 || kx == N / 2) {
+    const cx<double> P = sum_at(0, ky, kz), Pm = sum_at(0, my, mz);
+    if (kx == 0)
+      out = cx<double>{0.5 * (P.re + Pm.re), 0.5 * (P.im - Pm.im)};
+    else
+      out = cx<double>{0.5 * (P.im + Pm.im), 0.5 * (Pm.re - P.re)};
+  } else if (kx < N / 2) {
+    out = sum_at(kx, ky, kz);
+  } else {
+    const cx<double> t = sum_at(N - kx, my, mz);
+    out = cx<double>{t.re, -t.im};
+  }
+  acc64[gid] = out;
+}
+
 int collide_fused(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                   int with_loss, double hb, hipStream_t st, int* handled) {
-  (void)P; (void)f; (void)q; (void)batch; (void)pidx; (void)with_loss; (void)hb; (void)st;
+  using namespace f3;
   *handled = 0;
+  if (!(P->dim == 3 && P->n == 32 && P->precision == IB_PREC_FP32)) return IB_OK;
+  *handled = 1;
+  const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 256);
+  const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
+  const size_t part_bytes = (size_t)P->nr * 2 * NPL * PLANE * sizeof(cx<float>);
+  const size_t fh32_bytes = (size_t)17 * PLANE * sizeof(cx<float>) + (size_t)NPTS * sizeof(float);
+  int rc;
+  if ((rc = P->fhat.ensure(sub * b64))) return rc;
+  if ((rc = P->acc.ensure(sub * b64))) return rc;
+  if ((rc = P->work.ensure(2 * sub * b64))) return rc;
+  if ((rc = P->aux.ensure(sub * (part_bytes + fh32_bytes)))) return rc;
+  cx<float>* part = (cx<float>*)P->aux.ptr;
+  cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + sub * part_bytes);
+  float* f32 = (float*)((char*)fh32 + sub * 17 * PLANE * sizeof(cx<float>));
+  for (long long c0 = 0; c0 < batch; c0 += sub) {
+    const long long nb = std::min(sub, batch - c0);
+    const double* fc = f + c0 * NPTS;
+    if ((rc = staged_prologue_fp64_332(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
+    const long long tot = nb * NPTS;
+    hipLaunchKernelGGL(k_f3_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       (const cx<double>*)P->fhat.ptr, fc, fh32, f32, nb);
+    IB_HIP(hipGetLastError());
+    F3Args A{};
+    A.fh = fh32;
+    A.f32 = f32;
+    A.part = part;
+    A.pidx = pidx;
+    A.batch = nb;
+    A.cell0 = 0;
+    A.sigma = P->d_sigma;
+    A.rj = P->d_rj;
+    A.wjm = P->d_wjm;
+    A.a = P->d_a;
+    A.spg = (const float*)P->d_spg;
+    A.nr = P->nr;
+    A.m = P->m;
+    A.nq = P->nq;
+    A.L = P->L;
+    // pidx is indexed from the sub-batch start
+    A.pidx = pidx ? pidx + c0 : nullptr;
+    const long long groups = (nb + 7) / 8;
+    hipLaunchKernelGGL(k_fused3, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
+    IB_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_f3_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
+                       (cx<double>*)P->acc.ptr, nb, P->nr);
+    IB_HIP(hipGetLastError());
+    if ((rc = staged_epilogue_fp64_332(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr,
+                                       P->work.ptr, with_loss, hb, c0)))
+      return rc;
+  }
   return IB_OK;
 }
 

@@ -519,11 +519,55 @@ int collide_plain(Plan* P, const double* f, double* q, long long batch, const in
 }
 
 // ---------------------------------------------------------------------------
+// FP64 prologue / epilogue of the fused FP32 engine on 3D 32^3 grids.
+int staged_prologue_fp64_332(Plan* P, const double* f, long long nb, void* fhat64, void* work) {
+  hipStream_t st = P->stream;
+  BArgs<double> B{};
+  B.f = f;
+  B.w = (cx<double>*)work;
+  B.items = nb;
+  B.P = 1;
+  IB_TRY((launch_b<double, 32, 3, B_PRO>(B, st)));
+  AArgs<double> A{};
+  A.src = (const cx<double>*)work;
+  A.dst = (cx<double>*)fhat64;
+  A.items = nb;
+  IB_TRY((launch_a<double, 32, 3, A_OUT_PRO>(A, st)));
+  return IB_OK;
+}
+
+int staged_epilogue_fp64_332(Plan* P, const double* f, double* q, long long nb, void* acc64,
+                             void* fhat64, void* work, int with_loss, double hb, long long cell0) {
+  hipStream_t st = P->stream;
+  AArgs<double> A{};
+  A.src = (const cx<double>*)acc64;
+  A.src2 = (const cx<double>*)fhat64;
+  A.dst = (cx<double>*)work;
+  A.items = nb;
+  A.lam = P->d_lam64;
+  A.L = P->L;
+  A.hb = hb;
+  A.with_loss = with_loss;
+  A.cell0 = cell0;
+  IB_TRY((launch_a<double, 32, 3, A_IN_EPI>(A, st)));
+  BArgs<double> B{};
+  B.w = (cx<double>*)work;
+  B.f = f;
+  B.q = q;
+  B.items = nb;
+  B.P = 1;
+  B.with_loss = with_loss;
+  IB_TRY((launch_b<double, 32, 3, B_EPI>(B, st)));
+  return IB_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Table construction: sp(a_j r_j |k|) per (parameter set, j, |k|^2) and the
 // combined loss symbol Lambda(|k|^2) = sum_j |S| r_j^(gamma+d-1) w_j sp(pi r_j |k| / L).
 template <typename T>
-__global__ void k_tables(T* spg, T* lam, const double* a, const double* rj, const double* wr,
-                         int n_param, int nr, int nq, int dim, double L, double gamma, double S) {
+__global__ void k_tables(T* spg, T* lam, double* lam64, const double* a, const double* rj,
+                         const double* wr, int n_param, int nr, int nq, int dim, double L,
+                         double gamma, double S) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long ng = (long long)n_param * nr * nq;
   if (gid < ng) {
@@ -541,6 +585,7 @@ __global__ void k_tables(T* spg, T* lam, const double* a, const double* rj, cons
       s += lk * sp_func(dim, (M_PI * rj[j]) * kn / L);
     }
     lam[q] = (T)s;
+    if (lam64) lam64[q] = s;
   }
 }
 
@@ -549,11 +594,11 @@ int launch_tables(Plan* P, const double* d_wr, hipStream_t st) {
   const unsigned blocks = (unsigned)((total + 255) / 256);
   if (P->precision == IB_PREC_FP64)
     hipLaunchKernelGGL(k_tables<double>, dim3(blocks), dim3(256), 0, st, (double*)P->d_spg,
-                       (double*)P->d_lam, P->d_a, P->d_rj, d_wr, P->n_param, P->nr, P->nq, P->dim,
+                       (double*)P->d_lam, P->d_lam64, P->d_a, P->d_rj, d_wr, P->n_param, P->nr, P->nq, P->dim,
                        P->L, P->gamma, P->sphere_area);
   else
     hipLaunchKernelGGL(k_tables<float>, dim3(blocks), dim3(256), 0, st, (float*)P->d_spg,
-                       (float*)P->d_lam, P->d_a, P->d_rj, d_wr, P->n_param, P->nr, P->nq, P->dim,
+                       (float*)P->d_lam, P->d_lam64, P->d_a, P->d_rj, d_wr, P->n_param, P->nr, P->nq, P->dim,
                        P->L, P->gamma, P->sphere_area);
   IB_HIP(hipGetLastError());
   return IB_OK;

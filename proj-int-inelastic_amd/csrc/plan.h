@@ -45,9 +45,12 @@ struct Plan {
   double* d_a = nullptr;      // [n_param][nr]  pi (1 + e_j) / (4 L)
   void* d_spg = nullptr;      // [n_param][nr][nq]  sp(a_j r_j |k|), T
   void* d_lam = nullptr;      // [nq]  sum_j |S| r^(g+d-1) w_j sp(pi r_j |k| / L), T
+  double* d_lam64 = nullptr;  // [nq]  the same in FP64 (FP64 epilogue of the fused form)
   // device scratch (grow-only)
   DevBuf fhat, acc, work, aux;
   long long chunk_pairs = 0;  // pairs per staged chunk (0 = automatic)
+  long long fused_cells = 0;  // cells per fused launch (0 = automatic)
+  hipStream_t stream = nullptr;  // stream of the current call
   size_t scratch_budget = (size_t)1 << 30;
 };
 

@@ -192,3 +192,26 @@ def test_batch_invariance_fp64(pkg, need_gpu):
     for c in range(5):
         qc = coll.collide(torch.from_numpy(f[c]).cuda(), with_loss=True).cpu().numpy()
         assert _err(qc, qa[c]) <= 1e-13
+
+
+@pytest.mark.parametrize("with_loss,hb", [(True, 0.0), (False, 0.1)])
+def test_fused_fp32_3d32_batch(pkg, need_gpu, with_loss, hb):
+    """Fused FP32 form (3D 32^3): 11 cells with per-cell e (a partial XCD group),
+    launched in sub-batches of 4 cells, against the FP64 oracle cell by cell."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    d, N, nr, M, s = 3, 32, 4, 6, 3.624654714576
+    cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 1.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision="fp32")
+    coll.set_tuning(fused_cells=4)
+    es = np.linspace(0.5, 1.0, 11)
+    f = _data(d, N, sm.L, 11, seed=21)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=with_loss, heat_bath=hb, e=es).cpu().numpy()
+    worst = 0.0
+    for c in range(11):
+        op = _oracle(d, N, nr, M, s, es[c], es[c], 1.0)
+        ref = op(f[c], heat_bath=hb) if not with_loss else op.collide(f[c], with_loss=True)
+        worst = max(worst, _err(q[c], ref))
+    print("fused FP32 3D32 worst rel Linf", worst)
+    assert worst <= TOL32, worst
