@@ -149,6 +149,28 @@ class OracleOperator:
         self.loss = loss  # (Nr, 1^d)
         self.sp = sp_func(d, math.pi * r_loss * idx_norm / L)  # (Nr, N^d)
 
+    def _mirror(self, x):
+        """x(-k) on the FFT grid of the last d axes."""
+        d = self.d
+        out = x
+        for ax in range(-d, 0):
+            out = np.roll(np.flip(out, axis=ax), 1, axis=ax)
+        return out
+
+    def symmetrized(self):
+        """Operator with E and G replaced by their Hermitian parts (X + conj X(-k))/2.
+
+        Not the reference algorithm: this is the operator the real-data FP32 form
+        computes.  It equals the reference's C2C operator exactly when fhat
+        vanishes on the Nyquist planes, and differs by O(|fhat on the Nyquist
+        planes|) otherwise.
+        """
+        import copy
+        op = copy.copy(self)
+        op.exp = 0.5 * (self.exp + np.conj(self._mirror(self.exp)))
+        op.gain = 0.5 * (self.gain + np.conj(self._mirror(self.gain)))
+        return op
+
     # -- inelastic_collisions.py:208-227 (looped over r for bounded memory)
     def collide_parts(self, f, workers=1):
         """Return (gain_sum_hat_ifft, loss_sum) before the final scaling.

@@ -1,0 +1,283 @@
+"""Continuously batched solve server for many homogeneous and 1D-space problems.
+
+New (the reference solves one problem per Python process).  Problems are
+grouped into classes by their collision plan (dimension, grid, quadrature,
+kernel exponent, precision); a class keeps the distribution functions of its
+active problems resident in one device tensor.  Each scheduler tick advances
+every active problem by one outer step of its class's integrator, with the
+collision operator of all problems of the class (all spatial cells of all
+1D jobs, or all homogeneous problems) evaluated in one batched launch, each
+problem with its own restitution e.  Between ticks, finished problems retire
+(their f is copied to the host, the slot and memory are freed) and arrived
+requests are admitted while the HBM budget allows; a request that does not fit
+waits in the queue (back-pressure) instead of failing.
+
+Step of one tick, for a class with integrator "TPI2" and parameters
+(inner_steps, inner_dt, dt), is the reference's tel_proj_step2
+(src/boltzmann.py:130-140) applied to the stacked states; "PFE" is
+proj_euler_step (:120-128); "Euler" is euler_step (:82-84).  df of each
+problem is its TVD transport increment (1D jobs) plus dt Q(f)/kn.
+"""
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from . import device as dev
+from .collision import FSInelasticVHSCollision, SpectralMesh
+from .limiters import tvd
+
+
+@dataclass
+class Request:
+    """One solve: run f0 to t_final with the class's integrator.
+
+    dim/n/nr/m/s/gamma/precision select the collision plan; e is this
+    problem's restitution (e0 = e); kn the Knudsen number.  For a 1D job
+    f0 has shape (nx, n, n) and bc/xlen give the outflow boundaries and the
+    length of the x interval; homogeneous problems have f0 of shape n^dim.
+    """
+    rid: int
+    f0: np.ndarray
+    t_final: float
+    e: float
+    dim: int = 2
+    n: int = 32
+    nr: int = 32
+    m: int = 8
+    s: float = 3.624654714576
+    gamma: float = 1.0
+    kn: float = 1.0
+    precision: str = "fp64"
+    integrator: str = "TPI2"
+    inner_steps: tuple = (2, 2)
+    inner_dt: tuple = (0.01, 0.05)
+    dt: float = 0.25
+    xlen: float = 0.0  # > 0: 1D-space job on [0, xlen] with extrapolation boundaries
+    arrival: float = 0.0
+
+    def class_key(self):
+        return (self.dim, self.n, self.nr, self.m, self.s, self.gamma, self.precision,
+                self.integrator, tuple(self.inner_steps), tuple(self.inner_dt), self.dt)
+
+    @property
+    def cells(self):
+        return self.f0.shape[0] if self.xlen > 0 else 1
+
+    def nbytes(self):
+        # resident f plus per-step temporaries (df, Q, transport) in FP64
+        return 4 * self.f0.size * 8
+
+
+@dataclass
+class _Active:
+    req: Request
+    start: int      # first cell in the class tensor
+    cells: int
+    t: float = 0.0
+    steps_left: int = 0
+    admitted_at: float = 0.0
+
+
+@dataclass
+class Result:
+    rid: int
+    f: np.ndarray
+    t: float
+    arrival: float
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+class _Class:
+    """Problems sharing one collision plan and one integrator."""
+
+    def __init__(self, key, proto):
+        t = dev.require_gpu()
+        self.key = key
+        self.proto = proto
+        d = proto.dim
+        cfg = {"collision-model": {"dim": d, "e": 1.0, "e0": 1.0, "gamma": proto.gamma},
+               "velocity-mesh": {"nv": proto.n, "s": proto.s, "nr": proto.nr}}
+        if d == 2:
+            cfg["circle-quad-rule"] = {"nphi": proto.m}
+        else:
+            cfg["spherical-design-rule"] = {"ssrule": "lebedev", "nsphr": proto.m}
+        self.sm = SpectralMesh(cfg)
+        self.coll = FSInelasticVHSCollision(cfg, self.sm, precision=proto.precision)
+        self.vshape = (proto.n,) * d
+        self.active = []
+        self.f = t.empty((0,) + self.vshape, dtype=t.float64, device="cuda")
+        v = self.sm.v_center
+        self.v1 = dev.to_device(np.repeat(v[:, None], proto.n, axis=1).reshape(-1)) if d == 2 else None
+
+    def rebuild(self, keep, new):
+        """Compact the resident tensor to the kept problems and append new ones."""
+        t = dev.torch()
+        parts, act, start = [], [], 0
+        for a in keep:
+            parts.append(self.f[a.start:a.start + a.cells])
+            a.start = start
+            start += a.cells
+            act.append(a)
+        for a, f0 in new:
+            parts.append(dev.to_device(f0.reshape((a.cells,) + self.vshape)))
+            a.start = start
+            start += a.cells
+            act.append(a)
+        self.f = t.cat(parts).contiguous() if parts else self.f[:0]
+        self.active = act
+        cell_e = np.concatenate([np.full(a.cells, a.req.e) for a in act]) if act else np.zeros(0)
+        self.cell_e = cell_e
+        self.cell_kn = dev.to_device(np.concatenate([np.full(a.cells, a.req.kn) for a in act])) if act else None
+
+    # df of every problem of the class, stacked
+    def df(self, f, h):
+        t = dev.torch()
+        lib = _lib.load()
+        st = dev.stream_handle()
+        q = self.coll.collide(f, with_loss=True, e=self.cell_e)
+        out = t.empty_like(f)
+        for a in self.active:
+            sl = slice(a.start, a.start + a.cells)
+            adv = None
+            if a.req.xlen > 0:
+                adv = t.empty_like(f[sl])
+                dx = a.req.xlen / a.cells
+                _lib.check(lib.ib_tvd1d(f[sl].data_ptr(), adv.data_ptr(), a.cells, f[0].numel(),
+                                        self.v1.data_ptr(), float(h / dx), 2, 0, 1, 1,
+                                        self.proto.n, st), "ib_tvd1d")
+            _lib.check(lib.ib_df_combine(dev.ptr(adv), q[sl].data_ptr(), out[sl].data_ptr(), float(h),
+                                         float(a.req.kn), out[sl].numel(), st), "ib_df_combine")
+        return out
+
+    def step(self):
+        """One outer step of the class integrator on all active problems."""
+        p = self.proto
+        f = self.f
+        axpy = lambda y, x, a: _lib.check(_lib.load().ib_axpy(
+            y.data_ptr(), x.data_ptr(), float(a), y.numel(), dev.stream_handle()), "ib_axpy")
+        if p.integrator == "Euler":
+            axpy(f, self.df(f, p.dt), 1.0)
+        elif p.integrator == "PFE":
+            M = p.dt / p.inner_dt[0] - p.inner_steps[0]
+            d = None
+            for _ in range(p.inner_steps[0]):
+                d = self.df(f, p.inner_dt[0])
+                axpy(f, d, 1.0)
+            axpy(f, d, M)
+        elif p.integrator == "TPI2":
+            M1 = p.dt / p.inner_dt[1] - p.inner_steps[1]
+            d1 = None
+            for _ in range(p.inner_steps[1]):
+                M0 = p.inner_dt[1] / p.inner_dt[0] - p.inner_steps[0]
+                d0 = None
+                for _ in range(p.inner_steps[0]):
+                    d0 = self.df(f, p.inner_dt[0])
+                    axpy(f, d0, 1.0)
+                d1 = M0 * d0
+                axpy(f, d1, 1.0)
+            axpy(f, d1, M1)
+        else:
+            raise NotImplementedError("This time integrator is not implemented.")
+
+
+class Server:
+    """Admission by HBM budget, continuous batching, latency bookkeeping."""
+
+    def __init__(self, hbm_budget=64 << 30, max_cells_per_class=None, clock=time.perf_counter):
+        self.budget = int(hbm_budget)
+        self.used = 0
+        self.max_cells = max_cells_per_class
+        self.classes = {}
+        self.queue = []
+        self.results = {}
+        self.clock = clock
+        self.t0 = None
+
+    def submit(self, req):
+        self.queue.append(req)
+
+    def _now(self):
+        return self.clock() - self.t0
+
+    def _admit(self, now):
+        """Admit arrived requests in arrival order while memory allows."""
+        admitted = {}
+        waiting = []
+        for r in sorted(self.queue, key=lambda r: r.arrival):
+            if r.arrival > now:
+                waiting.append(r)
+                continue
+            cls = self.classes.get(r.class_key())
+            cells_now = sum(a.cells for a in cls.active) if cls else 0
+            cells_new = sum(a.cells for a, _ in admitted.get(r.class_key(), []))
+            fits_mem = self.used + r.nbytes() <= self.budget
+            fits_cls = self.max_cells is None or cells_now + cells_new + r.cells <= self.max_cells
+            if not (fits_mem and fits_cls):
+                waiting.append(r)
+                continue
+            n = int(round(r.t_final / r.dt))
+            if abs(n * r.dt - r.t_final) > 1e-5 * r.t_final:
+                raise Exception("dt does not divide (tend-tstart) and dt is fixed!")
+            a = _Active(req=r, start=0, cells=r.cells, steps_left=n, admitted_at=now)
+            admitted.setdefault(r.class_key(), []).append((a, r.f0))
+            self.used += r.nbytes()
+        self.queue = waiting
+        return admitted
+
+    def run(self, requests=None, realtime=True):
+        """Serve until every request is finished; returns {rid: Result}."""
+        for r in requests or []:
+            self.submit(r)
+        t = dev.require_gpu()
+        self.t0 = self.clock()
+        while self.queue or any(c.active for c in self.classes.values()):
+            now = self._now() if realtime else float("inf")
+            new = self._admit(now)
+            for key, items in new.items():
+                if key not in self.classes:
+                    self.classes[key] = _Class(key, items[0][0].req)
+                cls = self.classes[key]
+                cls.rebuild(cls.active, items)
+            busy = [c for c in self.classes.values() if c.active]
+            if not busy:
+                nxt = min(r.arrival for r in self.queue)
+                time.sleep(max(0.0, min(nxt - self._now(), 0.01)))
+                continue
+            for cls in busy:
+                cls.step()
+                for a in cls.active:
+                    a.steps_left -= 1
+                    a.t += cls.proto.dt
+            t.cuda.synchronize()
+            done_t = self._now()
+            for cls in busy:
+                done = [a for a in cls.active if a.steps_left <= 0]
+                if not done:
+                    continue
+                for a in done:
+                    fh = cls.f[a.start:a.start + a.cells].cpu().numpy().reshape(a.req.f0.shape)
+                    self.results[a.req.rid] = Result(a.req.rid, fh, a.t, a.req.arrival,
+                                                     a.admitted_at, done_t)
+                    self.used -= a.req.nbytes()
+                cls.rebuild([a for a in cls.active if a.steps_left > 0], [])
+        return self.results
+
+
+def latency_stats(results):
+    lat = np.array([r.latency for r in results.values()])
+    span = max(r.finished for r in results.values()) - min(r.arrival for r in results.values())
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "solves_per_s": len(lat) / span if span > 0 else float("inf"), "n": len(lat)}
+
+
+def poisson_arrivals(n, rate, rng):
+    """Arrival times of a Poisson process of the given rate (1/s)."""
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))

@@ -180,3 +180,23 @@ def test_planted_mistake_is_caught(monkeypatch):
     dfdt = (_bkw2(v2, K + h) - _bkw2(v2, K - h)) / (2 * h) * math.exp(-1 / 8) / 16
     Q = op.collide(_bkw2(v2, K), with_loss=True)
     assert np.abs(Q - dfdt).max() / np.abs(dfdt).max() > 1e-2
+
+
+def test_symmetrized_operator_equals_c2c_for_band_limited_data():
+    """The Hermitian-symmetrised operator (what the real-data FP32 form
+    computes) equals the reference C2C operator when fhat vanishes on every
+    Nyquist plane, and differs at first order in that content otherwise."""
+    sig, ws = Qd.lebedev_rule(14)
+    op = C.OracleOperator(3, 8, 2.0, 4, sig, ws, e=0.6, e0=0.9, gamma=1.0)
+    sy = op.symmetrized()
+    rng = np.random.default_rng(8)
+    f = rng.random((8, 8, 8))
+    fh = np.fft.fftn(f)
+    fh[4, :, :] = 0
+    fh[:, 4, :] = 0
+    fh[:, :, 4] = 0
+    fb = np.fft.ifftn(fh).real
+    q = op.collide(fb, with_loss=True)
+    assert np.abs(sy.collide(fb, with_loss=True) - q).max() < 1e-14 * np.abs(q).max()
+    q = op.collide(f, with_loss=True)
+    assert np.abs(sy.collide(f, with_loss=True) - q).max() > 1e-6 * np.abs(q).max()
