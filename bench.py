@@ -72,11 +72,13 @@ def _load_pkg():
     return mod
 
 
-def time_path(coll, f_dev, e, steps, warmup):
+def time_path(coll, f_dev, e, steps, warmup, timed_kernel=False):
     import torch
     for _ in range(warmup):
         coll.collide(f_dev, with_loss=True, e=e, project=True)
     torch.cuda.synchronize()
+    if timed_kernel:
+        coll.set_timing(True)
     st = torch.cuda.current_stream()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -92,20 +94,35 @@ def time_path(coll, f_dev, e, steps, warmup):
 
 
 def cpu_baseline(nb_sample, seed):
-    """FP64 oracle (scipy.fft, 8 worker threads) on a bounded sample of cells."""
+    """FP64 oracle (scipy.fft, 8 worker threads) on a bounded sample of cells.
+
+    One operator (one restitution value) serves every sampled cell: the work of
+    a Q evaluation does not depend on e.  Precomputation is excluded, as the
+    reference's plan building is.
+    """
     from oracle import collision as OC
     from oracle import quadrature as OQ
     sig, ws = OQ.lebedev_rule(M)
     L = 0.5 * (3 + math.sqrt(2)) * S3
     f, e = make_batch(nb_sample, L, seed)
     workers = 8
-    ops = [OC.OracleOperator(3, N, S3, NR, sig, ws, e=e[c], e0=e[c], gamma=1.0)
-           for c in range(nb_sample)]  # precomputation excluded, as the reference's setup
+    op = OC.OracleOperator(3, N, S3, NR, sig, ws, e=e[0], e0=e[0], gamma=1.0)
     t0 = time.perf_counter()
     for c in range(nb_sample):
-        ops[c].collide(f[c], with_loss=True, workers=workers)
+        op.collide(f[c], with_loss=True, workers=workers)
     dt = time.perf_counter() - t0
     return nb_sample / dt, workers, dt
+
+
+def traffic_from_profile():
+    """HBM bytes per fused launch from the committed PMC summary, or None."""
+    p = ROOT / "profiles" / "fused3_pmc_traffic.txt"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_cell"):
+            return float(line.split()[1])
+    return None
 
 
 def main():
@@ -116,7 +133,10 @@ def main():
     ap.add_argument("--cells", type=int, default=1024)
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--check-cells", type=int, default=2)
-    ap.add_argument("--cpu-cells", type=int, default=1)
+    ap.add_argument("--cpu-cells", type=int, default=40)
+    ap.add_argument("--mode", choices=["bench", "profile", "copy"], default="bench",
+                    help="profile: FP32 form only on --cells cells (for rocprofv3); "
+                         "copy: a plain 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
@@ -126,6 +146,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.mode == "copy":
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        torch.cuda.synchronize()
+        b = a.clone()
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
+        return
     pkg = _load_pkg()
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
 
@@ -134,7 +161,14 @@ def main():
     f, e = make_batch(args.cells, sm.L, seed=0)
     f_dev = torch.from_numpy(f).cuda()
     c32 = FSInelasticVHSCollision(cfg, sm, precision="fp32")
-    t32, w32, q32 = time_path(c32, f_dev, e, args.steps, args.warmup)
+    if args.mode == "profile":
+        for _ in range(args.warmup + args.steps):
+            c32.collide(f_dev, with_loss=True, e=e, project=True)
+        torch.cuda.synchronize()
+        print(json.dumps({"mode": "profile", "cells": args.cells, "launch_steps": args.warmup + args.steps}))
+        return
+    t32, w32, q32 = time_path(c32, f_dev, e, args.steps, args.warmup, timed_kernel=True)
+    kms, klaunch, kcells = c32.get_timing()
     res = {"fp32_s_per_step": t32}
     if not args.no_fp64:
         c64 = FSInelasticVHSCollision(cfg, sm, precision="fp64")
@@ -158,8 +192,12 @@ def main():
     mass_drift = float((mom[:, 0].abs() / fm[:, 0].abs()).max())
     cells_s = args.cells / t32
     flop = flops_per_cell()
-    achieved = flop * cells_s / 1e12
+    # dominant kernel (the fused FP32 kernel), timed live with HIP events on its stream
+    kernel_avg_ms = kms / max(klaunch, 1)
+    cells_per_launch = kcells / max(klaunch, 1)
+    achieved = flop * cells_per_launch / (kernel_avg_ms * 1e-3) / 1e12
     peak = 157.3
+    tr = traffic_from_profile()
     out = {
         "metric": "Q(f,f) cell-evals/s at N=32^3 (FP32 fast form, rel Linf vs FP64 oracle <= 1e-5)",
         "value": cells_s,
@@ -179,7 +217,11 @@ def main():
         "fp32_rel_linf_vs_oracle_max": max(errs) if errs else None,
         "fp32_mass_rel_drift_max": mass_drift,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None},
+                     "frac": achieved / peak,
+                     "traffic": tr * cells_per_launch if tr is not None else None,
+                     "kernel": "k_fused3", "kernel_avg_ms": kernel_avg_ms,
+                     "cells_per_launch": cells_per_launch,
+                     "flop_per_cell_alg": flop, "kernel_share_of_step": kms / (t32 * 1e3 * args.steps)},
     }
     if "fp64_s_per_step" in res:
         out["fp64_plain_cell_evals_per_s"] = args.cells / res["fp64_s_per_step"]

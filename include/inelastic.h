@@ -66,6 +66,14 @@ int ib_plan_destroy(ib_plan* plan);
 int ib_plan_set_tuning(ib_plan* plan, long long chunk_pairs, long long fused_cells,
                        long long scratch_budget);
 
+/* HIP-event timing of the dominant kernel of ib_collide: the fused FP32 kernel
+ * (cells counted per launch) or, on the staged path, the slab pass (cell x
+ * quadrature-pair items counted per launch).  set_timing clears the record;
+ * get_timing waits for the recorded events and returns the summed time.
+ * New (measurement hooks; the reference has no counterpart). */
+int ib_plan_set_timing(ib_plan* plan, int enable);
+int ib_plan_get_timing(ib_plan* plan, double* ms_total, long long* launches, long long* units);
+
 /* Q(f, f) for `batch` cells: f, q are (batch, n^dim) float64 device arrays.
  * param_idx: device int32 (batch,) parameter-set index per cell, or NULL for 0.
  * with_loss = 0 returns the gain term only (the reference default);

@@ -48,6 +48,8 @@ SIGNATURES = {
     "ib_plan_set_params": (_I, [_P, _I, _c.POINTER(_D), _P]),
     "ib_plan_destroy": (_I, [_P]),
     "ib_plan_set_tuning": (_I, [_P, _LL, _LL, _LL]),
+    "ib_plan_set_timing": (_I, [_P, _I]),
+    "ib_plan_get_timing": (_I, [_P, _c.POINTER(_D), _c.POINTER(_LL), _c.POINTER(_LL)]),
     "ib_collide": (_I, [_P, _P, _P, _LL, _P, _I, _D, _P]),
     "ib_laplacian": (_I, [_P, _P, _P, _LL, _P]),
     "ib_moments": (_I, [_P, _P, _LL, _I, _I, _D, _P]),

@@ -102,6 +102,17 @@ class FSInelasticVHSCollision(BaseCollision):
         _lib.check(_lib.load().ib_plan_set_tuning(self._plan, int(chunk_pairs), int(fused_cells),
                                                   int(scratch_budget)), "ib_plan_set_tuning")
 
+    def set_timing(self, enable=True):
+        """Record HIP events around the dominant kernel of every collide call."""
+        _lib.check(_lib.load().ib_plan_set_timing(self._plan, int(bool(enable))), "ib_plan_set_timing")
+
+    def get_timing(self):
+        """(milliseconds, launches, units) summed over the recorded launches."""
+        ms, nl, nu = ctypes.c_double(), ctypes.c_longlong(), ctypes.c_longlong()
+        _lib.check(_lib.load().ib_plan_get_timing(self._plan, ctypes.byref(ms), ctypes.byref(nl),
+                                                  ctypes.byref(nu)), "ib_plan_get_timing")
+        return ms.value, nl.value, nu.value
+
     def set_param_sets(self, e_r):
         """Install restitution profiles (n_param, nr); cells select one by index."""
         e_r = np.ascontiguousarray(e_r, dtype=np.float64)

@@ -23,6 +23,32 @@ int DevBuf::ensure(size_t want) {
   return IB_OK;
 }
 
+int Plan::timed_begin(hipStream_t st) {
+  if (!timing || ev_start.size() >= 8192) return IB_OK;
+  hipEvent_t a, b;
+  IB_HIP(hipEventCreate(&a));
+  IB_HIP(hipEventCreate(&b));
+  IB_HIP(hipEventRecord(a, st));
+  ev_start.push_back(a);
+  ev_stop.push_back(b);
+  ev_cells.push_back(0);
+  return IB_OK;
+}
+
+void Plan::timed_end(hipStream_t st, long long cells) {
+  if (!timing || ev_stop.empty() || ev_cells.back() != 0) return;
+  (void)hipEventRecord(ev_stop.back(), st);
+  ev_cells.back() = cells;
+}
+
+static void clear_timing(Plan* p) {
+  for (auto e : p->ev_start) (void)hipEventDestroy(e);
+  for (auto e : p->ev_stop) (void)hipEventDestroy(e);
+  p->ev_start.clear();
+  p->ev_stop.clear();
+  p->ev_cells.clear();
+}
+
 void DevBuf::release() {
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
@@ -187,9 +213,43 @@ int ib_plan_set_tuning(ib_plan* p, long long chunk_pairs, long long fused_cells,
   return IB_OK;
 }
 
+int ib_plan_set_timing(ib_plan* p, int enable) {
+  if (!p) {
+    set_error("null plan");
+    return IB_ERR_ARG;
+  }
+  IB_HIP(hipDeviceSynchronize());
+  clear_timing(p);
+  p->timing = enable ? 1 : 0;
+  return IB_OK;
+}
+
+int ib_plan_get_timing(ib_plan* p, double* ms_total, long long* launches, long long* cells) {
+  if (!p || !ms_total || !launches || !cells) {
+    set_error("null argument");
+    return IB_ERR_ARG;
+  }
+  double ms = 0;
+  long long nl = 0, nc = 0;
+  for (size_t i = 0; i < p->ev_start.size(); ++i) {
+    if (p->ev_cells[i] == 0) continue;
+    IB_HIP(hipEventSynchronize(p->ev_stop[i]));
+    float t = 0.f;
+    IB_HIP(hipEventElapsedTime(&t, p->ev_start[i], p->ev_stop[i]));
+    ms += t;
+    nl += 1;
+    nc += p->ev_cells[i];
+  }
+  *ms_total = ms;
+  *launches = nl;
+  *cells = nc;
+  return IB_OK;
+}
+
 int ib_plan_destroy(ib_plan* p) {
   if (!p) return IB_OK;
   (void)hipDeviceSynchronize();
+  clear_timing(p);
   free_param_tables(p);
   if (p->d_lam) (void)hipFree(p->d_lam);
   if (p->d_lam64) (void)hipFree(p->d_lam64);

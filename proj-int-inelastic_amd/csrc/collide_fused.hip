@@ -379,8 +379,10 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
     // pidx is indexed from the sub-batch start
     A.pidx = pidx ? pidx + c0 : nullptr;
     const long long groups = (nb + 7) / 8;
+    if ((rc = P->timed_begin(st))) return rc;
     hipLaunchKernelGGL(k_fused3, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
+    P->timed_end(st, nb);
     hipLaunchKernelGGL(k_f3_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
                        (cx<double>*)P->acc.ptr, nb, P->nr);
     IB_HIP(hipGetLastError());

@@ -472,7 +472,9 @@ static int collide_staged(Plan* P, const double* f, double* q, long long batch, 
       B.w = wk;
       B.items = nb * np;
       B.P = np;
+      IB_TRY(P->timed_begin(st));
       IB_TRY((launch_b<T, N, D, B_PAIR>(B, st)));
+      P->timed_end(st, nb * np);
       A.src = wk;
       A.dst = ac;
       A.items = nb;

@@ -51,6 +51,12 @@ struct Plan {
   long long chunk_pairs = 0;  // pairs per staged chunk (0 = automatic)
   long long fused_cells = 0;  // cells per fused launch (0 = automatic)
   hipStream_t stream = nullptr;  // stream of the current call
+  // optional HIP-event timing of the dominant kernel (k_fused3 / staged slab pass)
+  int timing = 0;
+  std::vector<hipEvent_t> ev_start, ev_stop;
+  std::vector<long long> ev_cells;
+  int timed_begin(hipStream_t st);
+  void timed_end(hipStream_t st, long long cells);
   size_t scratch_budget = (size_t)1 << 30;
 };
 
