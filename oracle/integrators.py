@@ -95,3 +95,34 @@ def evolve(f, df, integrator, dt, t0, tend, inner_steps=(3, 3, 3), inner_dt=None
         if callback is not None:
             callback(i, t, f)
     return f, t
+
+
+def convex_splitting_step(f, collision, df_advection, dt, kn, penalty):
+    """boltzmann.py:86-94: f += dt/(kn + p dt) Q(f); f += df_advection(f, dt)."""
+    f += dt / (kn + penalty * dt) * collision(f)
+    f += df_advection(f, dt)
+    return f
+
+
+def maxwellian_2d(vx, vy, rho, u, T):
+    """boltzmann.py:9-13 (per x-cell, u along v_x)."""
+    import numpy as np
+    rho = rho[:, None, None]
+    u = u[:, None, None]
+    T = T[:, None, None]
+    return rho / (2 * np.pi * T) * np.exp(-((vx - u) ** 2 + vy ** 2) / (2 * T))
+
+
+def convex_splitting_maxwellian_step(f, collision, df_advection, dt, kn, penalty, e, tau, vx, vy, dv):
+    """boltzmann.py:96-118 with state.py:31-52 moments (2D velocity, f (Nx, N, N))."""
+    import numpy as np
+    sum_f = lambda g: np.sum(g, axis=(-1, -2)) * dv * dv
+    rho = sum_f(f)
+    u = [sum_f(f * c) / rho for c in (vx, vy)]
+    E = 0.5 * sum_f(f * (vx ** 2 + vy ** 2))
+    T = E / rho - 0.5 * (u[0] ** 2 + u[1] ** 2)
+    T_new = (T - 8 * tau / (1 - e ** 2)) * np.exp(-(1 - e ** 2) / kn / 4 * rho * dt) + 8 * tau / (1 - e ** 2)
+    dm = maxwellian_2d(vx, vy, rho, u[0], T_new) - maxwellian_2d(vx, vy, rho, u[0], T)
+    f += dt / (kn + penalty * dt) * collision(f) + penalty * dt / (kn + penalty * dt) * dm
+    f += df_advection(f, dt)
+    return f

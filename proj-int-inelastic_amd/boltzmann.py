@@ -125,8 +125,9 @@ class BoltzmannSolver(Solver):
             e2 = 1 - self.e ** 2
             T_new = (T - 8 * self.tau / e2) * t.exp(-e2 / self.kn / 4 * rho * self.dt) + 8 * self.tau / e2
             dm = (maxwellian(vx, vy, rho, u, T_new) - maxwellian(vx, vy, rho, u, T)).reshape(state.f.shape)
-            c = self.dt / (self.kn + self.penalty * self.dt)
-            upd = c * self.collision(state.f) + self.penalty * c * dm
+            c1 = self.dt / (self.kn + self.penalty * self.dt)
+            c2 = self.penalty * self.dt / (self.kn + self.penalty * self.dt)
+            upd = c1 * self.collision(state.f) + c2 * dm
             self._axpy(state.f, upd, 1.0)
         adv = self.df_advection(state, self.dt)
         if adv is not None:

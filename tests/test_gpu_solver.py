@@ -210,3 +210,39 @@ def test_graph_replay_matches_eager(pkg, need_gpu):
         results.append(snaps)
     for a, b in zip(*results):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("integ", ["CONSP", "CONSP_M"])
+def test_convex_splitting_steps(pkg, need_gpu, integ):
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    nx, N, e, kn = 40, 16, 0.8, 1e-2
+    cfg = _coll_cfg(N, 16, 8, 3.0, e, 0.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, with_loss=True)
+    sol, s = _solver1d(pkg, coll, nx, N, sm.L, kn, integ, [2], [1e-2], 0.01)
+    s.penalty = 5.0
+    s.e = e
+    s.tau = 0.0
+    f0 = _sod(sm.v_center, nx)
+    sol.state.f = f0.copy()
+    s.evolve_to_time(sol, 0.02)
+    fg = sol.state.get_f_global()
+    op = OC.OracleOperator(2, N, 3.0, 16, *OQ.circle_rule(8), e=e, e0=e, gamma=0.0)
+    v = sm.v_center
+    v1 = np.repeat(v[:, None], N, axis=1)
+    dx = 1.0 / nx
+    adv = lambda f, h: OT.df_advection(f, v1, h, dx, OT.EXTRAP, OT.EXTRAP, 2, OT.minmod)
+    col = lambda f: op.collide(f, with_loss=True)
+    f = f0.copy()
+    vx = np.broadcast_to(v1, f.shape)
+    vy = np.broadcast_to(v1.T, f.shape)
+    for _ in range(2):
+        if integ == "CONSP":
+            f = OI.convex_splitting_step(f, col, adv, 0.01, kn, 5.0)
+        else:
+            f = OI.convex_splitting_maxwellian_step(f, col, adv, 0.01, kn, 5.0, e, 0.0, vx, vy, sm.delta)
+    err = np.abs(fg - f).max() / np.abs(f).max()
+    assert err < 1e-12, err
+    with pytest.raises(ValueError):
+        s.penalty = None
+        s.evolve_to_time(sol)
