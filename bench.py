@@ -125,6 +125,60 @@ def traffic_from_profile():
     return None
 
 
+def serve_run(precision, nreq, rate, seed=5):
+    """Poisson arrivals of homogeneous 2D/3D cooling solves and 1D x 2D jobs.
+
+    Homogeneous: 2D 16^2 / 32^2 (M = 8) and 3D 16^3 (M = 38 Lebedev), Nr = N,
+    Maxwell or hard-sphere kernel, e ~ U[0.2, 1], two-Gaussian data, TPI2 with
+    inner steps (2, 2), dt = (0.01, 0.05, 0.25), t_final in {0.5, 1, 1.5, 2}.
+    1D jobs: Nx in {64, 128}, 32^2 velocity, eps = 1e-2, hard spheres,
+    e ~ U[0.5, 1], Sod-type Maxwellian Riemann data, PFE (2 inner steps of
+    2.5e-4, CFL <= 0.26), dt = 1e-3, t_final in {0.01, 0.02}.  Returns latency
+    statistics.
+    """
+    from proj_int_inelastic_amd import serving
+    rng = np.random.default_rng(seed)
+    arrivals = serving.poisson_arrivals(nreq, rate, rng)
+    reqs = []
+    for i, t_arr in enumerate(arrivals):
+        kind = rng.integers(0, 4)
+        e = float(rng.uniform(0.2, 1.0))
+        gamma = float(rng.integers(0, 2))
+        if kind < 3:
+            d, n = [(2, 16), (2, 32), (3, 16)][kind]
+            m = 8 if d == 2 else 38
+            s = 3.0
+            L = 0.5 * (3 + math.sqrt(2)) * s
+            dv = 2 * L / n
+            v = -L + (np.arange(n) + 0.5) * dv
+            g = np.meshgrid(*([v] * d), indexing="ij")
+            c = rng.normal(0, 0.1, d)
+            f0 = sum(np.exp(-sum((g[a] - sg * (1.0 if a == 0 else 0.0) - c[a]) ** 2 for a in range(d)) / 1.0)
+                     for sg in (1, -1))
+            f0 /= f0.sum() * dv ** d
+            reqs.append(serving.Request(rid=i, f0=f0, t_final=float(rng.choice([0.5, 1.0, 1.5, 2.0])), e=e,
+                                        dim=d, n=n, nr=n, m=m, s=s, gamma=gamma, precision=precision,
+                                        integrator="TPI2", inner_steps=(2, 2), inner_dt=(0.01, 0.05),
+                                        dt=0.25, arrival=float(t_arr)))
+        else:
+            nx, n, s = int(rng.choice([64, 128])), 32, S3
+            L = 0.5 * (3 + math.sqrt(2)) * s
+            v = -L + (np.arange(n) + 0.5) * 2 * L / n
+            vx, vy = np.meshgrid(v, v, indexing="ij")
+            x = (np.arange(nx) + 0.5) / nx
+            rho = np.where(x < 0.5, 1.0, 0.125)[:, None, None]
+            T = np.where(x < 0.5, 1.0, 0.8)[:, None, None]
+            f0 = rho / (2 * math.pi * T) * np.exp(-(vx ** 2 + vy ** 2) / (2 * T))
+            reqs.append(serving.Request(rid=i, f0=f0, t_final=float(rng.choice([0.01, 0.02])),
+                                        e=float(rng.uniform(0.5, 1.0)), dim=2, n=n, nr=n, m=8, s=s,
+                                        gamma=1.0, kn=1e-2, precision=precision, integrator="PFE",
+                                        inner_steps=(2,), inner_dt=(2.5e-4,), dt=1e-3, xlen=1.0,
+                                        arrival=float(t_arr)))
+    srv = serving.Server(hbm_budget=32 << 30)
+    res = srv.run(reqs, realtime=True)
+    return serving.latency_stats(res)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +193,9 @@ def main():
                          "copy: a plain 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="requests of the Poisson-arrival serving run (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="arrival rate (1/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the Q(f,f) batch is sharded by cells with no "
@@ -227,6 +284,15 @@ def main():
         out["fp64_plain_cell_evals_per_s"] = args.cells / res["fp64_s_per_step"]
         out["fp32_over_fp64"] = res["fp64_s_per_step"] / t32
         out["fp32_vs_fp64_gpu_rel_linf_max"] = res["fp32_vs_fp64_gpu_rel_linf_max"]
+    if args.serve_requests > 0:
+        st32 = serve_run("fp32", args.serve_requests, args.serve_rate)
+        out["serving"] = {"fp32": st32,
+                          "workload": "Poisson arrivals (rate %.0f/s) of %d solves: homogeneous 2D 16^2/32^2, "
+                                      "3D 16^3 TPI2 cooling + 1D x 2D PFE jobs" % (args.serve_rate,
+                                                                                   args.serve_requests)}
+        if not args.no_fp64:
+            out["serving"]["fp64"] = serve_run("fp64", args.serve_requests, args.serve_rate)
+        out["p99_solve_latency_s"] = st32["p99_s"]
     if not args.no_cpu:
         rate, cores, dt = cpu_baseline(args.cpu_cells, seed=1)
         out["cpu_baseline"] = {"value": rate, "unit": "cell-evals/s", "cores": cores, "kind": "port",
