@@ -35,7 +35,7 @@ typedef struct ib_plan ib_plan;
  * e_j = (e0 - e)/2 tanh(r_j - R/2) + (e0 + e)/2 per parameter set. */
 typedef struct {
   int dim;        /* 2 or 3 */
-  int n;          /* points per velocity axis: 8, 16 or 32 */
+  int n;          /* points per velocity axis: 8, 16, 32, and 64 in 2D */
   int nr;         /* radial nodes */
   int m;          /* circle / sphere nodes */
   double L;       /* velocity box half width */
@@ -53,9 +53,12 @@ typedef struct {
  * Replaces src/collision/inelastic_collisions.py:57-137 (perform_precomputation). */
 int ib_plan_create(const ib_plan_desc* desc, ib_plan** out);
 
-/* Replace the restitution parameter sets (n_param x nr profile) of a plan.
- * Replaces re-running inelastic_collisions.py:72-103 for a new e. */
-int ib_plan_set_params(ib_plan* plan, int n_param, const double* e_r, void* stream);
+/* Replace the parameter sets of a plan: (n_param x nr) restitution profiles and,
+ * optionally, one VHS exponent per set (gamma = NULL keeps the plan's gamma).
+ * Cells select a set through ib_collide's param_idx.
+ * Replaces re-running inelastic_collisions.py:30-103 for a new e / gamma. */
+int ib_plan_set_params(ib_plan* plan, int n_param, const double* e_r, const double* gamma,
+                       void* stream);
 
 int ib_plan_destroy(ib_plan* plan);
 

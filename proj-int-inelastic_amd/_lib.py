@@ -45,7 +45,7 @@ _LL = _c.c_longlong
 
 SIGNATURES = {
     "ib_plan_create": (_I, [_c.POINTER(ib_plan_desc), _c.POINTER(_P)]),
-    "ib_plan_set_params": (_I, [_P, _I, _c.POINTER(_D), _P]),
+    "ib_plan_set_params": (_I, [_P, _I, _c.POINTER(_D), _c.POINTER(_D), _P]),
     "ib_plan_destroy": (_I, [_P]),
     "ib_plan_set_tuning": (_I, [_P, _LL, _LL, _LL]),
     "ib_plan_set_timing": (_I, [_P, _I]),

@@ -14,6 +14,8 @@ Extensions (keyword-only, all defaulting to the reference behaviour):
   with_loss   constructor keyword: make ``__call__`` return gain - loss.
   e, e0       per-cell restitution arrays for ``collide`` (batched problems
               with different e; e0 defaults to e, i.e. constant restitution).
+  gamma       per-cell VHS exponent array for ``collide`` (Maxwell and hard
+              spheres mixed in one batched launch).
   project     apply the FP64 conservative projection (mass, momentum) to Q.
 """
 import ctypes
@@ -113,29 +115,35 @@ class FSInelasticVHSCollision(BaseCollision):
                                                   ctypes.byref(nu)), "ib_plan_get_timing")
         return ms.value, nl.value, nu.value
 
-    def set_param_sets(self, e_r):
-        """Install restitution profiles (n_param, nr); cells select one by index."""
+    def set_param_sets(self, e_r, gamma=None):
+        """Install parameter sets: restitution profiles (n_param, nr) and optional
+        VHS exponents (n_param,); cells select one set by index."""
         e_r = np.ascontiguousarray(e_r, dtype=np.float64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        g = None if gamma is None else np.ascontiguousarray(gamma, dtype=np.float64)
         _lib.check(_lib.load().ib_plan_set_params(
-            self._plan, e_r.shape[0], e_r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-            dev.stream_handle()), "ib_plan_set_params")
+            self._plan, e_r.shape[0], e_r.ctypes.data_as(dp),
+            None if g is None else g.ctypes.data_as(dp), dev.stream_handle()), "ib_plan_set_params")
         self.e_r = e_r
 
-    def _param_index(self, nb, e, e0):
-        """Per-cell restitution -> (device index tensor or None)."""
+    def _param_index(self, nb, e, e0, gamma=None):
+        """Per-cell (e, e0, gamma) -> device parameter-set index tensor (or None)."""
         t = dev.torch()
-        if e is None:
+        if e is None and gamma is None:
             if self._param_key != ("config",):
                 self.set_param_sets(self.e_profile(self._e, self._e0)[None, :])
                 self._param_key = ("config",)
             return None
+        if e is None:
+            e, e0 = self._e, self._e0
         e = np.broadcast_to(np.asarray(e, dtype=np.float64), (nb,))
         e0 = e if e0 is None else np.broadcast_to(np.asarray(e0, dtype=np.float64), (nb,))
-        pairs = np.stack([e, e0], axis=1)
-        uniq, inv = np.unique(pairs, axis=0, return_inverse=True)
+        g = np.broadcast_to(np.asarray(self._gamma if gamma is None else gamma, dtype=np.float64), (nb,))
+        triples = np.stack([e, e0, g], axis=1)
+        uniq, inv = np.unique(triples, axis=0, return_inverse=True)
         key = ("cells", uniq.tobytes())
         if self._param_key != key:
-            self.set_param_sets(np.stack([self.e_profile(a, b) for a, b in uniq]))
+            self.set_param_sets(np.stack([self.e_profile(a, b) for a, b, _ in uniq]), uniq[:, 2])
             self._param_key = key
         return t.from_numpy(inv.reshape(-1).astype(np.int32)).to("cuda")
 
@@ -152,11 +160,12 @@ class FSInelasticVHSCollision(BaseCollision):
             return out
         return out.cpu().numpy()
 
-    def collide(self, input_f, with_loss=False, heat_bath=None, e=None, e0=None, project=False):
+    def collide(self, input_f, with_loss=False, heat_bath=None, e=None, e0=None, project=False,
+                gamma=None):
         lib = _lib.load()
 
         def go(f, out, nb):
-            pidx = self._param_index(nb, e, e0)
+            pidx = self._param_index(nb, e, e0, gamma)
             _lib.check(lib.ib_collide(self._plan, f.data_ptr(), out.data_ptr(), nb,
                                       dev.ptr(pidx), int(bool(with_loss)),
                                       float(heat_bath or 0.0), dev.stream_handle()), "ib_collide")

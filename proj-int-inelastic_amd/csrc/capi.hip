@@ -79,24 +79,42 @@ static int upload(X** dst, const X* src, size_t count) {
 static void free_param_tables(Plan* p) {
   if (p->d_a) (void)hipFree(p->d_a);
   if (p->d_spg) (void)hipFree(p->d_spg);
+  if (p->d_crad) (void)hipFree(p->d_crad);
+  if (p->d_lam) (void)hipFree(p->d_lam);
+  if (p->d_lam64) (void)hipFree(p->d_lam64);
   p->d_a = nullptr;
   p->d_spg = nullptr;
+  p->d_crad = nullptr;
+  p->d_lam = nullptr;
+  p->d_lam64 = nullptr;
 }
 
-// a_j = pi (1 + e_j) / (4 L) per parameter set (inelastic_collisions.py:80),
-// then the sp(a_j r_j |k|) table on the device.
-static int set_params(Plan* p, int n_param, const double* e_r, hipStream_t st, const double* d_wr) {
+// Per parameter set p = (restitution profile e_j, VHS exponent gamma_p):
+// a_j = pi (1 + e_j) / (4 L) (inelastic_collisions.py:80), the radial gain weight
+// r_j^(gamma_p + d - 1) w_j (:97-103 with |S| divided out, :225-227), and on the
+// device the sp(a_j r_j |k|) and loss-symbol tables.
+static int set_params(Plan* p, int n_param, const double* e_r, const double* gamma, hipStream_t st,
+                      const double* d_wr) {
   free_param_tables(p);
   p->n_param = n_param;
   p->e_r.assign(e_r, e_r + (size_t)n_param * p->nr);
-  std::vector<double> a((size_t)n_param * p->nr);
-  for (size_t i = 0; i < a.size(); ++i) a[i] = 0.25 * M_PI * (1 + p->e_r[i]) / p->L;
+  p->gamma_p.assign((size_t)n_param, p->gamma);
+  if (gamma)
+    for (int i = 0; i < n_param; ++i) p->gamma_p[i] = gamma[i];
+  std::vector<double> a((size_t)n_param * p->nr), crad((size_t)n_param * p->nr);
+  for (int q = 0; q < n_param; ++q)
+    for (int j = 0; j < p->nr; ++j) {
+      const size_t i = (size_t)q * p->nr + j;
+      a[i] = 0.25 * M_PI * (1 + p->e_r[i]) / p->L;
+      crad[i] = std::pow(p->r[j], p->gamma_p[q] + p->dim - 1) * p->wr[j];
+    }
   int rc = upload(&p->d_a, a.data(), a.size());
+  if (!rc) rc = upload(&p->d_crad, crad.data(), crad.size());
   if (rc) return rc;
   const size_t tsz = p->precision == IB_PREC_FP64 ? sizeof(double) : sizeof(float);
   IB_HIP(hipMalloc(&p->d_spg, (size_t)n_param * p->nr * p->nq * tsz));
-  if (!p->d_lam) IB_HIP(hipMalloc(&p->d_lam, (size_t)p->nq * tsz));
-  if (!p->d_lam64) IB_HIP(hipMalloc((void**)&p->d_lam64, (size_t)p->nq * sizeof(double)));
+  IB_HIP(hipMalloc(&p->d_lam, (size_t)n_param * p->nq * tsz));
+  IB_HIP(hipMalloc((void**)&p->d_lam64, (size_t)n_param * p->nq * sizeof(double)));
   rc = launch_tables(p, d_wr, st);
   if (rc) return rc;
   return IB_OK;
@@ -128,8 +146,8 @@ int ib_plan_create(const ib_plan_desc* d, ib_plan** out) {
     set_error("Dimension must be 2 or 3.");
     return IB_ERR_ARG;
   }
-  if (d->n != 8 && d->n != 16 && d->n != 32) {
-    set_error("n must be 8, 16 or 32");
+  if (!(d->n == 8 || d->n == 16 || d->n == 32 || (d->n == 64 && d->dim == 2))) {
+    set_error("n must be 8, 16 or 32 (3D) or 8 .. 64 (2D)");
     return IB_ERR_UNSUPPORTED;
   }
   if (d->nr < 1 || d->m < 1 || d->n_param < 1 || !(d->L > 0)) {
@@ -160,19 +178,11 @@ int ib_plan_create(const ib_plan_desc* d, ib_plan** out) {
   p->sigma.assign(d->sigma, d->sigma + (size_t)d->m * d->dim);
   p->wsigma.assign(d->wsigma, d->wsigma + d->m);
   p->sphere_area = 2 * std::pow(M_PI, 0.5 * p->dim) / std::tgamma(0.5 * p->dim);
-  // gain weight r^(gamma+d-1) w_r w_sigma: the reference's |S| r^(gamma+d-1) w_r w_sigma
-  // (inelastic_collisions.py:97-103) divided by |S| (:225-227)
-  std::vector<double> wjm((size_t)p->nr * p->m);
-  for (int j = 0; j < p->nr; ++j)
-    for (int m = 0; m < p->m; ++m) {
-      const double pre = p->sphere_area * std::pow(p->r[j], p->gamma + p->dim - 1);
-      wjm[(size_t)j * p->m + m] = pre * p->wr[j] * p->wsigma[m] / p->sphere_area;
-    }
   int rc = upload(&p->d_sigma, p->sigma.data(), p->sigma.size());
   if (!rc) rc = upload(&p->d_rj, p->r.data(), p->r.size());
   if (!rc) rc = upload(&p->d_wr, p->wr.data(), p->wr.size());
-  if (!rc) rc = upload(&p->d_wjm, wjm.data(), wjm.size());
-  if (!rc) rc = set_params(p, d->n_param, d->e_r, nullptr, p->d_wr);
+  if (!rc) rc = upload(&p->d_wsig, p->wsigma.data(), p->wsigma.size());
+  if (!rc) rc = set_params(p, d->n_param, d->e_r, nullptr, nullptr, p->d_wr);
   if (!rc) {
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) {
@@ -188,14 +198,15 @@ int ib_plan_create(const ib_plan_desc* d, ib_plan** out) {
   return IB_OK;
 }
 
-int ib_plan_set_params(ib_plan* p, int n_param, const double* e_r, void* stream) {
+int ib_plan_set_params(ib_plan* p, int n_param, const double* e_r, const double* gamma,
+                       void* stream) {
   if (!p || !e_r || n_param < 1) {
     set_error("invalid parameter sets");
     return IB_ERR_ARG;
   }
   hipStream_t st = (hipStream_t)stream;
   IB_HIP(hipStreamSynchronize(st));  // old tables may still be in use
-  int rc = set_params(p, n_param, e_r, st, p->d_wr);
+  int rc = set_params(p, n_param, e_r, gamma, st, p->d_wr);
   if (rc) return rc;
   IB_HIP(hipStreamSynchronize(st));
   return IB_OK;
@@ -251,12 +262,10 @@ int ib_plan_destroy(ib_plan* p) {
   (void)hipDeviceSynchronize();
   clear_timing(p);
   free_param_tables(p);
-  if (p->d_lam) (void)hipFree(p->d_lam);
-  if (p->d_lam64) (void)hipFree(p->d_lam64);
   if (p->d_sigma) (void)hipFree(p->d_sigma);
   if (p->d_rj) (void)hipFree(p->d_rj);
   if (p->d_wr) (void)hipFree(p->d_wr);
-  if (p->d_wjm) (void)hipFree(p->d_wjm);
+  if (p->d_wsig) (void)hipFree(p->d_wsig);
   p->fhat.release();
   p->acc.release();
   p->work.release();

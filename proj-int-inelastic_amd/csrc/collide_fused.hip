@@ -38,7 +38,8 @@ namespace ib {
 // FP64 staged-engine pieces (collide_plain.hip)
 int staged_prologue_fp64_332(Plan* P, const double* f, long long nb, void* fhat64, void* work);
 int staged_epilogue_fp64_332(Plan* P, const double* f, double* q, long long nb, void* acc64,
-                             void* fhat64, void* work, int with_loss, double hb, long long cell0);
+                             void* fhat64, void* work, int with_loss, double hb, long long cell0,
+                             const int* pidx);
 
 namespace f3 {
 constexpr int N = 32;
@@ -58,7 +59,8 @@ struct F3Args {
   long long cell0;
   const double* sigma;
   const double* rj;
-  const double* wjm;
+  const double* wsig;
+  const double* crad;
   const double* a;
   const float* spg;
   int nr, m, nq;
@@ -120,6 +122,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   const int ps = A.pidx ? A.pidx[A.cell0 + cell] : 0;
   for (int q = tid; q < A.nq; q += THREADS) sp[q] = A.spg[((long long)ps * A.nr + j) * A.nq + q];
   const double aj = A.a[ps * A.nr + j] * A.rj[j];
+  const double crad_j = A.crad[ps * A.nr + j];
   const double rpl = M_PI * A.rj[j] / A.L;
   const int p = tid / N, l = tid % N;                 // plane-pass role
   const int xy = tid / 16, xz = 2 * (tid % 16);        // x-pass role: pencils (xy, xz), (xy, xz+1)
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       sincos(th, &sn, &cs);
       (which ? tG : tE)[ax][k] = cx<float>{(float)cs, (float)sn};
     }
-    const float wj = (float)A.wjm[j * A.m + mm];
+    const float wj = (float)(crad_j * A.wsig[mm]);
     __syncthreads();
 
     cx<float> v[N];
@@ -369,7 +372,8 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
     A.cell0 = 0;
     A.sigma = P->d_sigma;
     A.rj = P->d_rj;
-    A.wjm = P->d_wjm;
+    A.wsig = P->d_wsig;
+    A.crad = P->d_crad;
     A.a = P->d_a;
     A.spg = (const float*)P->d_spg;
     A.nr = P->nr;
@@ -387,7 +391,7 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
                        (cx<double>*)P->acc.ptr, nb, P->nr);
     IB_HIP(hipGetLastError());
     if ((rc = staged_epilogue_fp64_332(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr,
-                                       P->work.ptr, with_loss, hb, c0)))
+                                       P->work.ptr, with_loss, hb, 0, pidx ? pidx + c0 : nullptr)))
       return rc;
   }
   return IB_OK;

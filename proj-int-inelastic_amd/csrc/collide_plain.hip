@@ -47,7 +47,8 @@ struct AArgs {
   long long cell0;
   const double* sigma;
   const double* rj;
-  const double* wjm;
+  const double* wsig;
+  const double* crad;
   const double* a;
   const T* spg;
   const T* lam;
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(64) void k_axis0(AArgs<T> A) {
       if (!active) continue;
       cx<T> gr = tab[sub][1][k1];
       if (D == 3) gr = gr * tab[sub][2][k2];
-      const T wj = (T)A.wjm[j * A.m + mm];
+      const T wj = (T)(A.crad[ps * A.nr + j] * A.wsig[mm]);
       const T* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
       const cx<T>* s = A.src + (cell * A.P + pl) * NPTS + p;
 #pragma unroll
@@ -169,6 +170,8 @@ __global__ __launch_bounds__(64) void k_axis0(AArgs<T> A) {
   } else {  // A_IN_EPI
     if (!active) return;
     const long long cell = item;
+    const int ps = A.pidx ? A.pidx[A.cell0 + cell] : 0;
+    const T* lam = A.lam + (long long)ps * A.nq;
     const cx<T>* ac = A.src + cell * NPTS + p;
     const cx<T>* fh = A.src2 + cell * NPTS + p;
     const double lap_c = -(M_PI * M_PI) / (A.L * A.L);
@@ -190,7 +193,7 @@ __global__ __launch_bounds__(64) void k_axis0(AArgs<T> A) {
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         const int ki = kfreq(i, N);
-        v[i] = (inv_n * A.lam[ki * ki + qrest]) * fh[(long long)i * NP];
+        v[i] = (inv_n * lam[ki * ki + qrest]) * fh[(long long)i * NP];
       }
       fft_reg<N, +1>(v);
       cx<T>* d2 = A.dst + (2 * cell + 1) * NPTS + p;
@@ -385,7 +388,7 @@ static int launch_a(const AArgs<T>& A, hipStream_t st) {
 
 template <typename T, int N, int D, int MODE>
 static int launch_b(const BArgs<T>& B, hipStream_t st) {
-  if (D == 2) {
+  if constexpr (D == 2) {
     const long long threads = B.items * N;
     hipLaunchKernelGGL((k_rest2<T, N, MODE>), dim3((unsigned)((threads + 63) / 64)), dim3(64), 0, st,
                        B);
@@ -434,7 +437,8 @@ static int collide_staged(Plan* P, const double* f, double* q, long long batch, 
     AArgs<T> A{};
     A.sigma = P->d_sigma;
     A.rj = P->d_rj;
-    A.wjm = P->d_wjm;
+    A.wsig = P->d_wsig;
+    A.crad = P->d_crad;
     A.a = P->d_a;
     A.spg = (const T*)P->d_spg;
     A.lam = (const T*)P->d_lam;
@@ -504,6 +508,7 @@ static int collide_staged_dispatch(Plan* P, const double* f, double* q, long lon
     case 208: return collide_staged<T, 8, 2>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
     case 216: return collide_staged<T, 16, 2>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
     case 232: return collide_staged<T, 32, 2>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
+    case 264: return collide_staged<T, 64, 2>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
     case 308: return collide_staged<T, 8, 3>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
     case 316: return collide_staged<T, 16, 3>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
     case 332: return collide_staged<T, 32, 3>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
@@ -539,7 +544,8 @@ int staged_prologue_fp64_332(Plan* P, const double* f, long long nb, void* fhat6
 }
 
 int staged_epilogue_fp64_332(Plan* P, const double* f, double* q, long long nb, void* acc64,
-                             void* fhat64, void* work, int with_loss, double hb, long long cell0) {
+                             void* fhat64, void* work, int with_loss, double hb, long long cell0,
+                             const int* pidx) {
   hipStream_t st = P->stream;
   AArgs<double> A{};
   A.src = (const cx<double>*)acc64;
@@ -547,6 +553,8 @@ int staged_epilogue_fp64_332(Plan* P, const double* f, double* q, long long nb, 
   A.dst = (cx<double>*)work;
   A.items = nb;
   A.lam = P->d_lam64;
+  A.nq = P->nq;
+  A.pidx = pidx;
   A.L = P->L;
   A.hb = hb;
   A.with_loss = with_loss;
@@ -568,8 +576,8 @@ int staged_epilogue_fp64_332(Plan* P, const double* f, double* q, long long nb, 
 // combined loss symbol Lambda(|k|^2) = sum_j |S| r_j^(gamma+d-1) w_j sp(pi r_j |k| / L).
 template <typename T>
 __global__ void k_tables(T* spg, T* lam, double* lam64, const double* a, const double* rj,
-                         const double* wr, int n_param, int nr, int nq, int dim, double L,
-                         double gamma, double S) {
+                         const double* crad, int n_param, int nr, int nq, int dim, double L,
+                         double S) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long ng = (long long)n_param * nr * nq;
   if (gid < ng) {
@@ -578,30 +586,32 @@ __global__ void k_tables(T* spg, T* lam, double* lam64, const double* a, const d
     const int j = (int)(pj % nr);
     const double x = (a[pj] * rj[j]) * sqrt((double)q);
     spg[gid] = (T)sp_func(dim, x);
-  } else if (gid < ng + nq) {
-    const int q = (int)(gid - ng);
+  } else if (gid < ng + (long long)n_param * nq) {
+    const long long pq = gid - ng;
+    const int ps = (int)(pq / nq), q = (int)(pq % nq);
     const double kn = sqrt((double)q);
     double s = 0.0;
     for (int j = 0; j < nr; ++j) {
-      const double lk = S * pow(rj[j], gamma + dim - 1) * wr[j];
+      const double lk = S * crad[(long long)ps * nr + j];
       s += lk * sp_func(dim, (M_PI * rj[j]) * kn / L);
     }
-    lam[q] = (T)s;
-    if (lam64) lam64[q] = s;
+    lam[pq] = (T)s;
+    lam64[pq] = s;
   }
 }
 
 int launch_tables(Plan* P, const double* d_wr, hipStream_t st) {
-  const long long total = (long long)P->n_param * P->nr * P->nq + P->nq;
+  (void)d_wr;
+  const long long total = (long long)P->n_param * (P->nr + 1) * P->nq;
   const unsigned blocks = (unsigned)((total + 255) / 256);
   if (P->precision == IB_PREC_FP64)
     hipLaunchKernelGGL(k_tables<double>, dim3(blocks), dim3(256), 0, st, (double*)P->d_spg,
-                       (double*)P->d_lam, P->d_lam64, P->d_a, P->d_rj, d_wr, P->n_param, P->nr, P->nq, P->dim,
-                       P->L, P->gamma, P->sphere_area);
+                       (double*)P->d_lam, P->d_lam64, P->d_a, P->d_rj, P->d_crad, P->n_param, P->nr,
+                       P->nq, P->dim, P->L, P->sphere_area);
   else
     hipLaunchKernelGGL(k_tables<float>, dim3(blocks), dim3(256), 0, st, (float*)P->d_spg,
-                       (float*)P->d_lam, P->d_lam64, P->d_a, P->d_rj, d_wr, P->n_param, P->nr, P->nq, P->dim,
-                       P->L, P->gamma, P->sphere_area);
+                       (float*)P->d_lam, P->d_lam64, P->d_a, P->d_rj, P->d_crad, P->n_param, P->nr,
+                       P->nq, P->dim, P->L, P->sphere_area);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }

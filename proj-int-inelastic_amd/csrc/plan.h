@@ -41,11 +41,13 @@ struct Plan {
   // device tables
   double* d_sigma = nullptr;  // [m][dim]
   double* d_rj = nullptr;     // [nr]
-  double* d_wjm = nullptr;    // [nr][m]  r^(gamma+d-1) w_r w_sigma
+  double* d_wsig = nullptr;   // [m]  sphere weights w_sigma
+  double* d_crad = nullptr;   // [n_param][nr]  r_j^(gamma_p + d - 1) w_j (gain weight / w_sigma)
   double* d_a = nullptr;      // [n_param][nr]  pi (1 + e_j) / (4 L)
   void* d_spg = nullptr;      // [n_param][nr][nq]  sp(a_j r_j |k|), T
-  void* d_lam = nullptr;      // [nq]  sum_j |S| r^(g+d-1) w_j sp(pi r_j |k| / L), T
-  double* d_lam64 = nullptr;  // [nq]  the same in FP64 (FP64 epilogue of the fused form)
+  void* d_lam = nullptr;      // [n_param][nq]  sum_j |S| r^(g+d-1) w_j sp(pi r_j |k| / L), T
+  double* d_lam64 = nullptr;  // [n_param][nq]  the same in FP64 (FP64 epilogue of the fused form)
+  std::vector<double> gamma_p;  // per parameter set
   // device scratch (grow-only)
   DevBuf fhat, acc, work, aux;
   long long chunk_pairs = 0;  // pairs per staged chunk (0 = automatic)

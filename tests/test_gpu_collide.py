@@ -83,6 +83,7 @@ CASES = [
     (2, 16, 8, 8, 2.0, 0.8, 0.8, 0.0),
     (2, 32, 32, 8, 2.718491035932, 0.8, 0.8, 0.0),  # homogeneous cooling shape
     (2, 32, 32, 8, 3.624654714576, 0.9, 0.9, 1.0),  # 1D x 2D shape (hard spheres)
+    (2, 64, 64, 8, 3.0, 0.4, 0.4, 1.0),  # 2D 64^2 of the uneven batch
     (3, 8, 4, 6, 2.0, 0.6, 0.95, 1.0),
     (3, 16, 8, 14, 3.0, 0.5, 0.5, 1.0),
     (3, 32, 6, 14, 3.624654714576, 0.75, 0.75, 1.0),
@@ -256,3 +257,21 @@ def test_fused_fp32_smooth_data_vs_c2c(pkg, need_gpu):
         err = _err(q[c], ref)
         print("fused FP32 smooth vs C2C", c, err)
         assert err <= TOL32, (c, err)
+
+
+@pytest.mark.parametrize("prec,d,N,tol", [("fp64", 2, 16, TOL64), ("fp32", 3, 32, TOL32)])
+def test_mixed_gamma_and_e_one_launch(pkg, need_gpu, prec, d, N, tol):
+    """Maxwell (gamma = 0) and hard-sphere (gamma = 1) cells with different e in one batch."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    nr, M, s = (8, 8, 2.5) if d == 2 else (4, 6, 3.624654714576)
+    cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 0.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision=prec)
+    es = np.array([0.3, 0.9, 0.3, 0.6])
+    gs = np.array([0.0, 1.0, 1.0, 0.0])
+    f = _smooth(d, N, sm.L, 4, seed=12)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True, e=es, gamma=gs).cpu().numpy()
+    for c in range(4):
+        ref = _oracle(d, N, nr, M, s, es[c], es[c], gs[c]).collide(f[c], with_loss=True)
+        assert _err(q[c], ref) <= tol, (c, _err(q[c], ref))
