@@ -59,7 +59,9 @@ class Request:
     arrival: float = 0.0
 
     def class_key(self):
-        return (self.dim, self.n, self.nr, self.m, self.s, self.gamma, self.precision,
+        # gamma and e are per-cell parameters of one plan: problems with either
+        # kernel share a class and its batched launches
+        return (self.dim, self.n, self.nr, self.m, self.s, self.precision,
                 self.integrator, tuple(self.inner_steps), tuple(self.inner_dt), self.dt)
 
     @property
@@ -135,6 +137,7 @@ class _Class:
         self.active = act
         cell_e = np.concatenate([np.full(a.cells, a.req.e) for a in act]) if act else np.zeros(0)
         self.cell_e = cell_e
+        self.cell_gamma = np.concatenate([np.full(a.cells, a.req.gamma) for a in act]) if act else np.zeros(0)
         self.cell_kn = dev.to_device(np.concatenate([np.full(a.cells, a.req.kn) for a in act])) if act else None
 
     # df of every problem of the class, stacked
@@ -142,7 +145,7 @@ class _Class:
         t = dev.torch()
         lib = _lib.load()
         st = dev.stream_handle()
-        q = self.coll.collide(f, with_loss=True, e=self.cell_e)
+        q = self.coll.collide(f, with_loss=True, e=self.cell_e, gamma=self.cell_gamma)
         out = t.empty_like(f)
         for a in self.active:
             sl = slice(a.start, a.start + a.cells)

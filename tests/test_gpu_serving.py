@@ -30,8 +30,8 @@ def _riemann(nx, N, L):
 
 def _requests(Request, L):
     reqs = []
-    for i, (e, tf) in enumerate([(0.3, 0.5), (0.9, 1.0), (0.6, 0.75), (1.0, 0.25)]):
-        reqs.append(Request(rid=i, f0=_gauss2(16, L, i), t_final=tf, e=e, n=16, nr=16, m=8, s=3.0,
+    for i, (e, tf, g) in enumerate([(0.3, 0.5, 0.0), (0.9, 1.0, 1.0), (0.6, 0.75, 1.0), (1.0, 0.25, 0.0)]):
+        reqs.append(Request(rid=i, f0=_gauss2(16, L, i), t_final=tf, e=e, gamma=g, n=16, nr=16, m=8, s=3.0,
                             integrator="TPI2", inner_steps=(2, 2), inner_dt=(0.01, 0.05), dt=0.25,
                             arrival=0.0))
     for i, e in enumerate([0.7, 0.95]):
