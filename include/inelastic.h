@@ -86,6 +86,16 @@ int ib_plan_get_timing(ib_plan* plan, double* ms_total, long long* launches, lon
 int ib_collide(ib_plan* plan, const double* f, double* q, long long batch,
                const int* param_idx, int with_loss, double heat_bath, void* stream);
 
+/* Grouped form of ib_collide for an uneven batch: nplans plans (any grid shapes,
+ * one precision), each with its own cells f[i], q[i], batch[i], param_idx[i]
+ * (param_idx may be NULL).  Every stage of the staged engine runs as ONE persistent
+ * kernel launch over the work of all plans (up to 8 plans per launch); FP32 plans
+ * whose shape has a fused kernel (3D 32^3) run it.  New: the reference evaluates
+ * one operator per call (src/collision/base.py:32-58). */
+int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, double* const* q,
+                     const long long* batch, const int* const* param_idx, int with_loss,
+                     double heat_bath, void* stream);
+
 /* Laplacian of f (spectral), out = Re ifft(-pi^2 |k|^2 / L^2 fft f).
  * Replaces src/collision/inelastic_collisions.py:229-230. */
 int ib_laplacian(ib_plan* plan, const double* f, double* out, long long batch, void* stream);

@@ -198,3 +198,35 @@ class FSInelasticVHSCollision(BaseCollision):
                                   dev.stream_handle()), "ib_moments")
         out = out.reshape(tuple(f.shape[:-d]) + (d + 2,))
         return out if is_dev else out.cpu().numpy()
+
+
+def collide_group(colls, fs, with_loss=False, heat_bath=None, es=None, gammas=None):
+    """Q(f, f) of an uneven batch in grouped launches (one launch per stage).
+
+    colls: operators of one precision (any grid shapes); fs: device tensors
+    (*B_i, N_i^d_i); es/gammas: optional per-cell arrays per operator.
+    Returns the list of Q tensors.
+    """
+    lib = _lib.load()
+    t = dev.torch()
+    n = len(colls)
+    outs, keep, pidx = [], [], []
+    nb = (ctypes.c_longlong * n)()
+    for i, (c, f) in enumerate(zip(colls, fs)):
+        f = dev.to_device(f)
+        keep.append(f)
+        d = c.ndim
+        nb[i] = int(np.prod(f.shape[:-d], dtype=np.int64)) if f.dim() > d else 1
+        outs.append(t.empty_like(f))
+        e = None if es is None else es[i]
+        g = None if gammas is None else gammas[i]
+        pi = c._param_index(nb[i], e, None, g)
+        keep.append(pi)
+        pidx.append(dev.ptr(pi))
+    P = ctypes.c_void_p
+    arr = lambda vals: (P * n)(*[P(v) for v in vals])
+    _lib.check(lib.ib_collide_group(
+        n, arr([c._plan.value for c in colls]), arr([f.data_ptr() for f in keep[0::2]]),
+        arr([o.data_ptr() for o in outs]), nb, arr(pidx), int(bool(with_loss)),
+        float(heat_bath or 0.0), dev.stream_handle()), "ib_collide_group")
+    return outs

@@ -1,6 +1,7 @@
 // C ABI (include/inelastic.h): argument checks, plan lifetime, dispatch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -60,6 +61,9 @@ int collide_plain(Plan* P, const double* f, double* q, long long batch, const in
 int collide_fused(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                   int with_loss, double hb, hipStream_t st, int* handled);
 int launch_tables(Plan* P, const double* d_wr, hipStream_t st);
+int collide_group(Plan** P, int np, const double* const* f, double* const* q,
+                  const long long* batch, const int* const* pidx, int with_loss, double hb,
+                  hipStream_t st);
 int launch_tvd1d(const double* f, double* df, int nx, long long nv, const double* v1, double dtdx,
                  int order, int limiter, int bc_lo, int bc_hi, hipStream_t st);
 int launch_axpy(double* y, const double* x, double a, long long n, hipStream_t st);
@@ -289,6 +293,54 @@ int ib_collide(ib_plan* p, const double* f, double* q, long long batch, const in
     if (rc || handled) return rc;
   }
   return collide_plain(p, f, q, batch, param_idx, with_loss, heat_bath, 0, st);
+}
+
+int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, double* const* q,
+                     const long long* batch, const int* const* param_idx, int with_loss,
+                     double heat_bath, void* stream) {
+  if (nplans < 1 || !plans || !f || !q || !batch) {
+    set_error("invalid grouped collide arguments");
+    return IB_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int prec = plans[0]->precision;
+  std::vector<Plan*> gp;
+  std::vector<const double*> gf;
+  std::vector<double*> gq;
+  std::vector<long long> gb;
+  std::vector<const int*> gi;
+  for (int i = 0; i < nplans; ++i) {
+    ib_plan* p = plans[i];
+    if (!p || batch[i] < 0 || (batch[i] > 0 && (!f[i] || !q[i]))) {
+      set_error("invalid grouped collide arguments");
+      return IB_ERR_ARG;
+    }
+    if (p->precision != prec) {
+      set_error("all plans of a grouped launch must have the same precision");
+      return IB_ERR_ARG;
+    }
+    if (batch[i] == 0) continue;
+    p->stream = st;
+    const int* pi = param_idx ? param_idx[i] : nullptr;
+    if (prec == IB_PREC_FP32) {  // shapes with a fused FP32 kernel run it
+      int handled = 0;
+      int rc = collide_fused(p, f[i], q[i], batch[i], pi, with_loss, heat_bath, st, &handled);
+      if (rc) return rc;
+      if (handled) continue;
+    }
+    gp.push_back(p);
+    gf.push_back(f[i]);
+    gq.push_back(q[i]);
+    gb.push_back(batch[i]);
+    gi.push_back(pi);
+  }
+  for (size_t s0 = 0; s0 < gp.size(); s0 += 8) {
+    const int n = (int)std::min<size_t>(8, gp.size() - s0);
+    int rc = collide_group(gp.data() + s0, n, gf.data() + s0, gq.data() + s0, gb.data() + s0,
+                           gi.data() + s0, with_loss, heat_bath, st);
+    if (rc) return rc;
+  }
+  return IB_OK;
 }
 
 int ib_laplacian(ib_plan* p, const double* f, double* out, long long batch, void* stream) {

@@ -65,19 +65,35 @@ struct BArgs {
   int P, with_loss;
 };
 
-// ---------------------------------------------------------------------------
-// Axis-0 pass.  One thread per axis-0 pencil; 64-thread workgroups.
+// LDS bytes of one 64-thread block of each pass (the grouped kernel takes the
+// maximum over the shapes it serves).
 template <typename T, int N, int D, int MODE>
-__global__ __launch_bounds__(64) void k_axis0(AArgs<T> A) {
+struct Axis0Lds {
+  static constexpr int SUBS = Geo<N, D>::SUBS;
+  static constexpr size_t tab = sizeof(cx<T>) * SUBS * D * N;
+  static constexpr size_t bytes = tab + sizeof(cx<T>) * (MODE == A_OUT_PAIR ? N : 1) * 64;
+};
+
+template <typename T, int N>
+struct Rest3Lds {
+  static constexpr size_t bytes = sizeof(cx<T>) * (64 / N) * N * (N + 1);
+};
+
+// ---------------------------------------------------------------------------
+// Axis-0 pass.  One thread per axis-0 pencil; 64-thread blocks.  (bx, by) is
+// the block's position in the pass's 2D grid (pencil block, item group); lds
+// points to Axis0Lds<...>::bytes of workgroup memory.
+template <typename T, int N, int D, int MODE>
+__device__ __forceinline__ void axis0_body(const AArgs<T>& A, int bx, long long by, char* lds) {
   using G = Geo<N, D>;
   constexpr int NP = G::NP, NPW = G::NPW, SUBS = G::SUBS;
   constexpr long long NPTS = G::NPTS;
-  __shared__ cx<T> tab[SUBS][D][N];
-  __shared__ cx<T> accs[MODE == A_OUT_PAIR ? N : 1][64];
+  auto tab = reinterpret_cast<cx<T>(*)[D][N]>(lds);
+  auto accs = reinterpret_cast<cx<T>(*)[64]>(lds + Axis0Lds<T, N, D, MODE>::tab);
   const int tid = threadIdx.x;
   const int sub = tid / NPW, lane = tid % NPW;
-  const long long item = (long long)blockIdx.y * SUBS + sub;
-  const int p = blockIdx.x * NPW + lane;
+  const long long item = by * SUBS + sub;
+  const int p = bx * NPW + lane;
   const bool active = item < A.items;
   int k1 = 0, k2 = 0;
   if (D == 2) {
@@ -203,12 +219,18 @@ __global__ __launch_bounds__(64) void k_axis0(AArgs<T> A) {
   }
 }
 
+template <typename T, int N, int D, int MODE>
+__global__ __launch_bounds__(64) void k_axis0(AArgs<T> A) {
+  __shared__ __align__(16) char lds[Axis0Lds<T, N, D, MODE>::bytes];
+  axis0_body<T, N, D, MODE>(A, blockIdx.x, blockIdx.y, lds);
+}
+
 // ---------------------------------------------------------------------------
 // Remaining-axes pass, 2D: one thread per row (axis 1 pencil, contiguous).
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__(64) void k_rest2(BArgs<T> B) {
+__device__ __forceinline__ void rest2_body(const BArgs<T>& B, long long bx) {
   constexpr long long NPTS = (long long)N * N;
-  const long long gid = (long long)blockIdx.x * 64 + threadIdx.x;
+  const long long gid = bx * 64 + threadIdx.x;
   const long long item = gid / N;
   const int row = (int)(gid % N);
   if (item >= B.items) return;
@@ -259,19 +281,24 @@ __global__ __launch_bounds__(64) void k_rest2(BArgs<T> B) {
   }
 }
 
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(64) void k_rest2(BArgs<T> B) {
+  rest2_body<T, N, MODE>(B, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Remaining-axes pass, 3D: one N x N slab (fixed axis-0 index) per sub-group of
 // N threads, staged in LDS with rows padded to N + 1 (conflict-free columns
 // and rows).
 template <typename T, int N, int MODE>
-__global__ __launch_bounds__(64) void k_rest3(BArgs<T> B) {
+__device__ __forceinline__ void rest3_body(const BArgs<T>& B, long long bx, char* lds) {
   constexpr int NP = N * N;
   constexpr long long NPTS = (long long)N * NP;
   constexpr int SPW = 64 / N;
-  __shared__ cx<T> tile[SPW][N][N + 1];
+  auto tile = reinterpret_cast<cx<T>(*)[N][N + 1]>(lds);
   const int tid = threadIdx.x;
   const int s = tid / N, l = tid % N;
-  const long long gs = (long long)blockIdx.x * SPW + s;
+  const long long gs = bx * SPW + s;
   const long long item = gs / N;
   const int i0 = (int)(gs % N);
   const bool active = item < B.items;
@@ -374,6 +401,12 @@ __global__ __launch_bounds__(64) void k_rest3(BArgs<T> B) {
       }
     }
   }
+}
+
+template <typename T, int N, int MODE>
+__global__ __launch_bounds__(64) void k_rest3(BArgs<T> B) {
+  __shared__ __align__(16) char lds[Rest3Lds<T, N>::bytes];
+  rest3_body<T, N, MODE>(B, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -523,6 +556,250 @@ int collide_plain(Plan* P, const double* f, double* q, long long batch, const in
   if (P->precision == IB_PREC_FP64)
     return collide_staged_dispatch<double>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
   return collide_staged_dispatch<float>(P, f, q, batch, pidx, with_loss, hb, no_gain, st);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped persistent launch: one kernel per stage serves several plans of
+// different grid shapes (an uneven batch).  The segment table travels in the
+// kernel arguments; each 64-thread block walks a strided range of the
+// concatenated virtual blocks of all segments and dispatches on the segment's
+// shape to the same pass bodies the per-shape kernels use.
+constexpr int kMaxSeg = 8;
+
+template <typename T>
+struct GSeg {
+  int kind;
+  long long boff, blocks;
+  AArgs<T> A;
+  BArgs<T> B;
+};
+
+template <typename T>
+struct GSegs {
+  int nseg;
+  long long total;
+  GSeg<T> s[kMaxSeg];
+};
+
+#define IB_KINDS(X) X(8, 2) X(16, 2) X(32, 2) X(64, 2) X(8, 3) X(16, 3) X(32, 3)
+
+template <typename T, int MODE, bool PASS_A>
+__global__ __launch_bounds__(64) void k_group(GSegs<T> G) {
+  extern __shared__ __align__(16) char lds[];
+  for (long long vb = blockIdx.x; vb < G.total; vb += gridDim.x) {
+    __syncthreads();  // the previous item's LDS readers are done
+    int s = 0;
+    while (s + 1 < G.nseg && G.s[s + 1].boff <= vb) ++s;
+    const long long local = vb - G.s[s].boff;
+    if constexpr (PASS_A) {
+      switch (G.s[s].kind) {
+#define IB_CASE_A(NN, DD)                                                          \
+  case DD * 100 + NN: {                                                            \
+    constexpr int BLK = Geo<NN, DD>::BLK;                                          \
+    axis0_body<T, NN, DD, MODE>(G.s[s].A, (int)(local % BLK), local / BLK, lds);   \
+  } break;
+        IB_KINDS(IB_CASE_A)
+#undef IB_CASE_A
+        default:
+          break;
+      }
+    } else {
+      switch (G.s[s].kind) {
+        case 208: rest2_body<T, 8, MODE>(G.s[s].B, local); break;
+        case 216: rest2_body<T, 16, MODE>(G.s[s].B, local); break;
+        case 232: rest2_body<T, 32, MODE>(G.s[s].B, local); break;
+        case 264: rest2_body<T, 64, MODE>(G.s[s].B, local); break;
+        case 308: rest3_body<T, 8, MODE>(G.s[s].B, local, lds); break;
+        case 316: rest3_body<T, 16, MODE>(G.s[s].B, local, lds); break;
+        case 332: rest3_body<T, 32, MODE>(G.s[s].B, local, lds); break;
+        default: break;
+      }
+    }
+  }
+}
+
+// virtual blocks and LDS bytes of one segment of a pass
+template <typename T, int MODE, bool PASS_A>
+static void seg_shape(int kind, long long items, long long* blocks, size_t* lds) {
+  switch (kind) {
+#define IB_SHAPE(NN, DD)                                                                 \
+  case DD * 100 + NN:                                                                    \
+    if (PASS_A) {                                                                        \
+      using Gm = Geo<NN, DD>;                                                            \
+      *blocks = (long long)Gm::BLK * ((items + Gm::SUBS - 1) / Gm::SUBS);                \
+      *lds = Axis0Lds<T, NN, DD, MODE>::bytes;                                           \
+    } else if (DD == 2) {                                                                \
+      *blocks = (items * NN + 63) / 64;                                                  \
+      *lds = 0;                                                                          \
+    } else {                                                                             \
+      *blocks = (items * NN + (64 / NN) - 1) / (64 / NN);                                \
+      *lds = Rest3Lds<T, NN>::bytes;                                                     \
+    }                                                                                    \
+    break;
+    IB_KINDS(IB_SHAPE)
+#undef IB_SHAPE
+    default:
+      *blocks = 0;
+      *lds = 0;
+  }
+}
+
+template <typename T, int MODE, bool PASS_A>
+static int launch_group(GSegs<T>& G, hipStream_t st) {
+  size_t lds = 0;
+  long long total = 0;
+  for (int i = 0; i < G.nseg; ++i) {
+    long long nb;
+    size_t l;
+    seg_shape<T, MODE, PASS_A>(G.s[i].kind, PASS_A ? G.s[i].A.items : G.s[i].B.items, &nb, &l);
+    G.s[i].boff = total;
+    G.s[i].blocks = nb;
+    total += nb;
+    lds = std::max(lds, l);
+  }
+  G.total = total;
+  if (total == 0) return IB_OK;
+  static size_t attr_bytes = 0;
+  if (lds > 65536 && lds > attr_bytes) {
+    IB_HIP(hipFuncSetAttribute((const void*)k_group<T, MODE, PASS_A>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_bytes = lds;
+  }
+  const unsigned grid = (unsigned)std::min<long long>(total, 4096);
+  hipLaunchKernelGGL((k_group<T, MODE, PASS_A>), dim3(grid), dim3(64), lds, st, G);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+template <typename T>
+static int collide_group_t(Plan** P, int np, const double* const* f, double* const* q,
+                           const long long* batch, const int* const* pidx, int with_loss,
+                           double hb, hipStream_t st) {
+  struct S {
+    Plan* p;
+    long long nb, pc;
+    int npairs;
+    size_t arr;
+  };
+  std::vector<S> segs;
+  for (int i = 0; i < np; ++i) {
+    if (batch[i] == 0) continue;
+    const size_t arr = (size_t)P[i]->npts * sizeof(cx<T>);
+    long long pc = std::max<long long>(2, (long long)(P[i]->scratch_budget / (batch[i] * arr)));
+    const int npairs = P[i]->nr * P[i]->m;
+    pc = std::min<long long>(pc, std::max(npairs, 2));
+    IB_TRY(P[i]->fhat.ensure(batch[i] * arr));
+    IB_TRY(P[i]->acc.ensure(batch[i] * arr));
+    IB_TRY(P[i]->work.ensure((size_t)batch[i] * pc * arr));
+    segs.push_back(S{P[i], batch[i], pc, npairs, arr});
+  }
+  const int ns = (int)segs.size();
+  if (ns == 0) return IB_OK;
+  if (ns > kMaxSeg) {
+    set_error("at most 8 plans per grouped launch");
+    return IB_ERR_ARG;
+  }
+  // index of each non-empty segment in the caller's arrays
+  std::vector<int> idx;
+  for (int i = 0; i < np; ++i)
+    if (batch[i] != 0) idx.push_back(i);
+  auto base = [&](int k) {
+    Plan* p = segs[k].p;
+    GSeg<T> g{};
+    g.kind = p->dim * 100 + p->n;
+    AArgs<T>& A = g.A;
+    A.sigma = p->d_sigma;
+    A.rj = p->d_rj;
+    A.wsig = p->d_wsig;
+    A.crad = p->d_crad;
+    A.a = p->d_a;
+    A.spg = (const T*)p->d_spg;
+    A.lam = (const T*)p->d_lam;
+    A.nr = p->nr;
+    A.m = p->m;
+    A.nq = p->nq;
+    A.L = p->L;
+    A.hb = hb;
+    A.with_loss = with_loss;
+    A.pidx = pidx ? pidx[idx[k]] : nullptr;
+    A.cell0 = 0;
+    g.B.f = f[idx[k]];
+    g.B.q = q[idx[k]];
+    g.B.with_loss = with_loss;
+    return g;
+  };
+  GSegs<T> G{};
+  // prologue
+  G.nseg = ns;
+  for (int k = 0; k < ns; ++k) {
+    G.s[k] = base(k);
+    G.s[k].B.w = (cx<T>*)segs[k].p->work.ptr;
+    G.s[k].B.items = segs[k].nb;
+    G.s[k].B.P = 1;
+  }
+  IB_TRY((launch_group<T, B_PRO, false>(G, st)));
+  for (int k = 0; k < ns; ++k) {
+    G.s[k].A.src = (const cx<T>*)segs[k].p->work.ptr;
+    G.s[k].A.dst = (cx<T>*)segs[k].p->fhat.ptr;
+    G.s[k].A.items = segs[k].nb;
+  }
+  IB_TRY((launch_group<T, A_OUT_PRO, true>(G, st)));
+  // quadrature pairs: round r serves chunk r of every plan that has one
+  for (int r = 0;; ++r) {
+    GSegs<T> Gi{}, Gb{}, Go{};
+    int live = 0;
+    for (int k = 0; k < ns; ++k) {
+      const long long p0 = (long long)r * segs[k].pc;
+      if (p0 >= segs[k].npairs) continue;
+      const int npr = (int)std::min<long long>(segs[k].pc, segs[k].npairs - p0);
+      GSeg<T> g = base(k);
+      g.A.src = (const cx<T>*)segs[k].p->fhat.ptr;
+      g.A.dst = (cx<T>*)segs[k].p->work.ptr;
+      g.A.items = segs[k].nb * npr;
+      g.A.P = npr;
+      g.A.p0 = (int)p0;
+      Gi.s[live] = g;
+      g.B.w = (cx<T>*)segs[k].p->work.ptr;
+      g.B.items = segs[k].nb * npr;
+      g.B.P = npr;
+      Gb.s[live] = g;
+      g.A.src = (const cx<T>*)segs[k].p->work.ptr;
+      g.A.dst = (cx<T>*)segs[k].p->acc.ptr;
+      g.A.items = segs[k].nb;
+      g.A.first = (r == 0);
+      Go.s[live] = g;
+      ++live;
+    }
+    if (live == 0) break;
+    Gi.nseg = Gb.nseg = Go.nseg = live;
+    IB_TRY((launch_group<T, A_IN_PAIR, true>(Gi, st)));
+    IB_TRY((launch_group<T, B_PAIR, false>(Gb, st)));
+    IB_TRY((launch_group<T, A_OUT_PAIR, true>(Go, st)));
+  }
+  // epilogue
+  for (int k = 0; k < ns; ++k) {
+    G.s[k] = base(k);
+    G.s[k].A.src = (const cx<T>*)segs[k].p->acc.ptr;
+    G.s[k].A.src2 = (const cx<T>*)segs[k].p->fhat.ptr;
+    G.s[k].A.dst = (cx<T>*)segs[k].p->work.ptr;
+    G.s[k].A.items = segs[k].nb;
+  }
+  IB_TRY((launch_group<T, A_IN_EPI, true>(G, st)));
+  for (int k = 0; k < ns; ++k) {
+    G.s[k].B.w = (cx<T>*)segs[k].p->work.ptr;
+    G.s[k].B.items = segs[k].nb;
+    G.s[k].B.P = 1;
+  }
+  IB_TRY((launch_group<T, B_EPI, false>(G, st)));
+  return IB_OK;
+}
+
+int collide_group(Plan** P, int np, const double* const* f, double* const* q,
+                  const long long* batch, const int* const* pidx, int with_loss, double hb,
+                  hipStream_t st) {
+  if (P[0]->precision == IB_PREC_FP64)
+    return collide_group_t<double>(P, np, f, q, batch, pidx, with_loss, hb, st);
+  return collide_group_t<float>(P, np, f, q, batch, pidx, with_loss, hb, st);
 }
 
 // ---------------------------------------------------------------------------

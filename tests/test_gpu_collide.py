@@ -275,3 +275,37 @@ def test_mixed_gamma_and_e_one_launch(pkg, need_gpu, prec, d, N, tol):
     for c in range(4):
         ref = _oracle(d, N, nr, M, s, es[c], es[c], gs[c]).collide(f[c], with_loss=True)
         assert _err(q[c], ref) <= tol, (c, _err(q[c], ref))
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+def test_grouped_launch_matches_per_plan(pkg, need_gpu, prec, tol):
+    """One grouped launch per stage over plans of five grid shapes equals the
+    per-plan launches (and the oracle for the FP64 form)."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    from proj_int_inelastic_amd.collision.inelastic_collisions import collide_group
+    shapes = [(2, 16, 8, 8, 2.5, 3), (2, 32, 8, 8, 3.0, 2), (2, 64, 16, 8, 3.0, 1),
+              (3, 8, 4, 6, 2.0, 2), (3, 16, 4, 14, 3.0, 2)]
+    if prec == "fp32":
+        shapes.append((3, 32, 4, 6, 3.624654714576, 2))
+    colls, fs, es, gs, refs = [], [], [], [], []
+    rng = np.random.default_rng(31)
+    for d, N, nr, M, s, nb in shapes:
+        cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 0.0)
+        sm = SpectralMesh(cfg)
+        c = FSInelasticVHSCollision(cfg, sm, precision=prec)
+        f = _smooth(d, N, sm.L, nb, seed=N + d)
+        e = rng.uniform(0.3, 1.0, nb)
+        g = rng.integers(0, 2, nb).astype(float)
+        colls.append(c)
+        fs.append(torch.from_numpy(f).cuda())
+        es.append(e)
+        gs.append(g)
+    qs = [q.cpu().numpy() for q in collide_group(colls, fs, with_loss=True, es=es, gammas=gs)]
+    for i, (c, f) in enumerate(zip(colls, fs)):
+        qi = c.collide(f, with_loss=True, e=es[i], gamma=gs[i]).cpu().numpy()
+        assert _err(qs[i], qi) <= tol, (shapes[i], _err(qs[i], qi))
+    if prec == "fp64":
+        d, N, nr, M, s, nb = shapes[2]
+        ref = _oracle(d, N, nr, M, s, es[2][0], es[2][0], gs[2][0]).collide(fs[2][0].cpu().numpy(), with_loss=True)
+        assert _err(qs[2][0], ref) <= TOL64
