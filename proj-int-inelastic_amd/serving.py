@@ -27,7 +27,7 @@ import numpy as np
 from . import _lib
 from . import device as dev
 from .collision import FSInelasticVHSCollision, SpectralMesh
-from .limiters import tvd
+from .collision.inelastic_collisions import collide_group
 
 
 @dataclass
@@ -61,8 +61,12 @@ class Request:
     def class_key(self):
         # gamma and e are per-cell parameters of one plan: problems with either
         # kernel share a class and its batched launches
-        return (self.dim, self.n, self.nr, self.m, self.s, self.precision,
-                self.integrator, tuple(self.inner_steps), tuple(self.inner_dt), self.dt)
+        return (self.dim, self.n, self.nr, self.m, self.s) + self.group_key()
+
+    def group_key(self):
+        # classes with one integrator schedule step together (grouped launches)
+        return (self.precision, self.integrator, tuple(self.inner_steps), tuple(self.inner_dt),
+                self.dt)
 
     @property
     def cells(self):
@@ -135,17 +139,17 @@ class _Class:
             act.append(a)
         self.f = t.cat(parts).contiguous() if parts else self.f[:0]
         self.active = act
+        self.version = getattr(self, "version", 0) + 1
         cell_e = np.concatenate([np.full(a.cells, a.req.e) for a in act]) if act else np.zeros(0)
         self.cell_e = cell_e
         self.cell_gamma = np.concatenate([np.full(a.cells, a.req.gamma) for a in act]) if act else np.zeros(0)
         self.cell_kn = dev.to_device(np.concatenate([np.full(a.cells, a.req.kn) for a in act])) if act else None
 
-    # df of every problem of the class, stacked
-    def df(self, f, h):
+    def df_from_q(self, f, q, h):
+        """Stacked df of every problem: TVD transport (1D jobs) + h Q / kn."""
         t = dev.torch()
         lib = _lib.load()
         st = dev.stream_handle()
-        q = self.coll.collide(f, with_loss=True, e=self.cell_e, gamma=self.cell_gamma)
         out = t.empty_like(f)
         for a in self.active:
             sl = slice(a.start, a.start + a.cells)
@@ -160,21 +164,48 @@ class _Class:
                                          float(a.req.kn), out[sl].numel(), st), "ib_df_combine")
         return out
 
-    def step(self):
-        """One outer step of the class integrator on all active problems."""
-        p = self.proto
-        f = self.f
-        axpy = lambda y, x, a: _lib.check(_lib.load().ib_axpy(
-            y.data_ptr(), x.data_ptr(), float(a), y.numel(), dev.stream_handle()), "ib_axpy")
+
+def _axpy(y, x, a):
+    _lib.check(_lib.load().ib_axpy(y.data_ptr(), x.data_ptr(), float(a), y.numel(),
+                                   dev.stream_handle()), "ib_axpy")
+
+
+class _Group:
+    """Classes (grid shapes) sharing one integrator schedule.
+
+    Each df stage evaluates Q of every problem of every member class with one
+    grouped launch per engine stage (ib_collide_group).  While the membership
+    is unchanged, one whole outer step is replayed from a captured HIP graph.
+    """
+
+    def __init__(self, key, use_graph=True):
+        self.key = key
+        self.classes = []
+        self.use_graph = use_graph
+        self._graph = None
+        self._graph_key = None
+
+    def df(self, h):
+        cls = self.classes
+        qs = collide_group([c.coll for c in cls], [c.f for c in cls], with_loss=True,
+                           es=[c.cell_e for c in cls], gammas=[c.cell_gamma for c in cls])
+        return [c.df_from_q(c.f, q, h) for c, q in zip(cls, qs)]
+
+    def _outer_step(self):
+        p = self.classes[0].proto
+        fs = [c.f for c in self.classes]
         if p.integrator == "Euler":
-            axpy(f, self.df(f, p.dt), 1.0)
+            for f, d in zip(fs, self.df(p.dt)):
+                _axpy(f, d, 1.0)
         elif p.integrator == "PFE":
             M = p.dt / p.inner_dt[0] - p.inner_steps[0]
-            d = None
+            ds = None
             for _ in range(p.inner_steps[0]):
-                d = self.df(f, p.inner_dt[0])
-                axpy(f, d, 1.0)
-            axpy(f, d, M)
+                ds = self.df(p.inner_dt[0])
+                for f, d in zip(fs, ds):
+                    _axpy(f, d, 1.0)
+            for f, d in zip(fs, ds):
+                _axpy(f, d, M)
         elif p.integrator == "TPI2":
             M1 = p.dt / p.inner_dt[1] - p.inner_steps[1]
             d1 = None
@@ -182,19 +213,42 @@ class _Class:
                 M0 = p.inner_dt[1] / p.inner_dt[0] - p.inner_steps[0]
                 d0 = None
                 for _ in range(p.inner_steps[0]):
-                    d0 = self.df(f, p.inner_dt[0])
-                    axpy(f, d0, 1.0)
-                d1 = M0 * d0
-                axpy(f, d1, 1.0)
-            axpy(f, d1, M1)
+                    d0 = self.df(p.inner_dt[0])
+                    for f, d in zip(fs, d0):
+                        _axpy(f, d, 1.0)
+                d1 = [M0 * d for d in d0]
+                for f, d in zip(fs, d1):
+                    _axpy(f, d, 1.0)
+            for f, d in zip(fs, d1):
+                _axpy(f, d, M1)
         else:
             raise NotImplementedError("This time integrator is not implemented.")
+
+    def step(self):
+        t = dev.torch()
+        gkey = tuple((id(c), c.version) for c in self.classes)
+        if self.use_graph and self._graph is not None and self._graph_key == gkey:
+            self._graph.replay()
+            return
+        self._outer_step()
+        if not self.use_graph:
+            return
+        # membership changed: the eager step above sized the scratch and installed
+        # the parameter sets; capture the next steps' work for replay
+        t.cuda.synchronize()
+        g = t.cuda.CUDAGraph()
+        with t.cuda.graph(g):
+            self._outer_step()
+        self._graph, self._graph_key = g, gkey
 
 
 class Server:
     """Admission by HBM budget, continuous batching, latency bookkeeping."""
 
-    def __init__(self, hbm_budget=64 << 30, max_cells_per_class=None, clock=time.perf_counter):
+    def __init__(self, hbm_budget=64 << 30, max_cells_per_class=None, clock=time.perf_counter,
+                 use_graph=True):
+        self.use_graph = use_graph
+        self.groups = {}
         self.budget = int(hbm_budget)
         self.used = 0
         self.max_cells = max_cells_per_class
@@ -254,8 +308,11 @@ class Server:
                 nxt = min(r.arrival for r in self.queue)
                 time.sleep(max(0.0, min(nxt - self._now(), 0.01)))
                 continue
+            for gk in sorted({c.proto.group_key() for c in busy}, key=repr):
+                grp = self.groups.setdefault(gk, _Group(gk, self.use_graph))
+                grp.classes = [c for c in busy if c.proto.group_key() == gk]
+                grp.step()
             for cls in busy:
-                cls.step()
                 for a in cls.active:
                     a.steps_left -= 1
                     a.t += cls.proto.dt

@@ -65,3 +65,18 @@ def test_backpressure_queue(pkg, need_gpu):
     for k in res:
         assert np.abs(res[k].f - free[k].f).max() <= 1e-12 * np.abs(free[k].f).max()
     assert srv.used == 0
+
+
+def test_graph_replay_and_mixed_shapes(pkg, need_gpu):
+    """Grouped launches over mixed shapes with HIP-graph replay equal the eager loop."""
+    from proj_int_inelastic_amd import serving
+    L = 0.5 * (3 + math.sqrt(2)) * 3.0
+    reqs = []
+    for i, (n, e, g, tf) in enumerate([(16, 0.4, 0.0, 1.0), (32, 0.8, 1.0, 1.0), (16, 0.9, 1.0, 0.5)]):
+        reqs.append(serving.Request(rid=i, f0=_gauss2(n, L, i), t_final=tf, e=e, gamma=g, n=n, nr=n, m=8,
+                                    s=3.0, integrator="TPI2", inner_steps=(2, 2), inner_dt=(0.01, 0.05),
+                                    dt=0.25))
+    a = serving.Server(use_graph=True).run(list(reqs), realtime=False)
+    b = serving.Server(use_graph=False).run(list(reqs), realtime=False)
+    for k in a:
+        assert np.array_equal(a[k].f, b[k].f), k
