@@ -179,6 +179,49 @@ def serve_run(precision, nreq, rate, seed=5):
     return serving.latency_stats(res)
 
 
+def uneven_requests(nprob, precision, seed=9):
+    """The uneven batch of homogeneous cooling problems.
+
+    2D N in {16, 32, 64}^2 (M = 8) and 3D N in {16, 32}^3 (M = 38 Lebedev),
+    Nr = N, Maxwell or hard-sphere kernel at random, e ~ U[0.2, 1], seeded
+    two-Gaussian data, TPI2 with 2 levels of K = 2 inner steps
+    (dt0 = 0.01, dt1 = 0.05, outer dt = 0.25) to t = 2.
+    """
+    from proj_int_inelastic_amd import serving
+    rng = np.random.default_rng(seed)
+    shapes = [(2, 16, 8), (2, 32, 8), (2, 64, 8), (3, 16, 38), (3, 32, 38)]
+    reqs = []
+    for i in range(nprob):
+        d, n, m = shapes[int(rng.integers(0, len(shapes)))]
+        s = 3.0 if n < 32 else S3
+        L = 0.5 * (3 + math.sqrt(2)) * s
+        dv = 2 * L / n
+        v = -L + (np.arange(n) + 0.5) * dv
+        g = np.meshgrid(*([v] * d), indexing="ij")
+        c = rng.normal(0, 0.1, (2, d))
+        f0 = sum(np.exp(-sum((g[a] - sg * (1.0 if a == 0 else 0.0) - c[k, a]) ** 2
+                             for a in range(d)) / (2 * 0.5)) for k, sg in enumerate((1, -1)))
+        f0 /= f0.sum() * dv ** d
+        reqs.append(serving.Request(rid=i, f0=f0, t_final=2.0, e=float(rng.uniform(0.2, 1.0)), dim=d,
+                                    n=n, nr=n, m=m, s=s, gamma=float(rng.integers(0, 2)),
+                                    precision=precision, integrator="TPI2", inner_steps=(2, 2),
+                                    inner_dt=(0.01, 0.05), dt=0.25, arrival=0.0))
+    return reqs
+
+
+def uneven_run(precision, nprob):
+    import torch
+    from proj_int_inelastic_amd import serving
+    reqs = uneven_requests(nprob, precision)
+    srv = serving.Server(hbm_budget=200 << 30)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = srv.run(reqs, realtime=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"solves": len(res), "seconds": dt, "solves_per_s": len(res) / dt}, res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,6 +239,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the Poisson-arrival serving run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="arrival rate (1/s)")
+    ap.add_argument("--uneven-problems", type=int, default=256,
+                    help="problems of the uneven homogeneous batch (0 = skip; 4096 = full size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the Q(f,f) batch is sharded by cells with no "
@@ -284,6 +329,16 @@ def main():
         out["fp64_plain_cell_evals_per_s"] = args.cells / res["fp64_s_per_step"]
         out["fp32_over_fp64"] = res["fp64_s_per_step"] / t32
         out["fp32_vs_fp64_gpu_rel_linf_max"] = res["fp32_vs_fp64_gpu_rel_linf_max"]
+    if args.uneven_problems > 0:
+        u32, r32 = uneven_run("fp32", args.uneven_problems)
+        out["uneven_batch"] = {"workload": "homogeneous cooling, 2D 16^2/32^2/64^2 + 3D 16^3/32^3, "
+                                           "TPI2 to t=2, grouped launches", "fp32": u32}
+        if not args.no_fp64:
+            u64, r64 = uneven_run("fp64", args.uneven_problems)
+            out["uneven_batch"]["fp64"] = u64
+            out["uneven_batch"]["fp32_over_fp64"] = u32["solves_per_s"] / u64["solves_per_s"]
+            errs = [np.abs(r32[k].f - r64[k].f).max() / np.abs(r64[k].f).max() for k in r64]
+            out["uneven_batch"]["fp32_vs_fp64_final_f_rel_linf_max"] = float(max(errs))
     if args.serve_requests > 0:
         st32 = serve_run("fp32", args.serve_requests, args.serve_rate)
         out["serving"] = {"fp32": st32,
