@@ -106,12 +106,26 @@ __device__ __forceinline__ cx<float> sym_at(const SymRow& r, cx<float> fz, bool 
   return nz ? (r.nyq * fz).re * r.rest : r.row * fz;
 }
 
+// Pair schedule (4 workgroup barriers per quadrature pair):
+//   A  E * fhat and the inverse kz transform (pencil (p, ky) in registers); the
+//      packed plane p = 0 adds i * X16 staged by phase D of the previous pair
+//   B  inverse ky; all threads: G * (.) and the sum for the packed plane of the
+//      previous pair (2 elements each, read from SP with its (-ky, -kz) mirror)
+//   C  along x (C2R, * f, R2C); threads 0..191: phase tables of the next pair
+//   D  forward ky; all threads: X16 = E16 * fhat16 of the next pair
+//   E  forward kz, then G * (.) and the sum (p >= 1, registers) or the packed
+//      plane parked in SP (p = 0)
+// Tables are double-buffered so phase C can fill the next pair's while A..E of
+// the current pair read theirs; no thread's work depends on the packed plane's
+// special case beyond a short tail, so no wave runs two code paths in full.
 __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   using namespace f3;
   __shared__ cx<float> W[NPL][N][PAD];
-  __shared__ cx<float> tE[3][N];
-  __shared__ cx<float> tG[3][N];
+  __shared__ cx<float> tE[2][3][N];
+  __shared__ cx<float> tG[2][3][N];
   __shared__ float sp[3 * (N / 2) * (N / 2) + 1];
+  __shared__ cx<float> S16[N][PAD];
+  __shared__ cx<float> SP[N][PAD];
   const int tid = threadIdx.x;
   const long long id = blockIdx.x;
   const int c8 = (int)(id % 8);
@@ -131,6 +145,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   const int kyf = kfreq(l, N);
 
   cx<float> acc[N];
+  cx<float> acc2[2], cmp2[2];
 #if IB_FUSED_KAHAN
   cx<float> cmp[N];
 #else
@@ -140,9 +155,9 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   for (int k = 0; k < N; ++k) acc[k] = cx<float>{0.f, 0.f};
 #pragma unroll
   for (int k = 0; k < (int)(sizeof(cmp) / sizeof(cmp[0])); ++k) cmp[k] = cx<float>{0.f, 0.f};
+  acc2[0] = acc2[1] = cmp2[0] = cmp2[1] = cx<float>{0.f, 0.f};
 
-  for (int mm = 0; mm < A.m; ++mm) {
-    __syncthreads();
+  auto make_tables = [&](int mm, int b) {
     if (tid < 6 * N) {
       const int which = tid / (3 * N), ax = (tid % (3 * N)) / N, k = tid % N;
       const double s = A.sigma[mm * 3 + ax];
@@ -150,34 +165,64 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       const double th = which == 0 ? -(rpl * s * kf) : (aj * s * kf);
       double sn, cs;
       sincos(th, &sn, &cs);
-      (which ? tG : tE)[ax][k] = cx<float>{(float)cs, (float)sn};
+      (which ? tG : tE)[b][ax][k] = cx<float>{(float)cs, (float)sn};
     }
-    const float wj = (float)(crad_j * A.wsig[mm]);
-    __syncthreads();
+  };
+  auto make_x16 = [&](int b) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = tid + h * THREADS, ky = e / N, kz = e % N;
+      const SymRow r16 = sym_row(tE[b][0][N / 2], true, tE[b][1][ky], ky == N / 2);
+      S16[ky][kz] = sym_at(r16, tE[b][2][kz], kz == N / 2) * fh[16 * PLANE + e];
+    }
+  };
+  auto packed_acc = [&](int b, float wjp) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int e = tid + h * THREADS, ky = e / N, kz = e % N;
+      const cx<float> P = SP[ky][kz], Pm = SP[(N - ky) % N][(N - kz) % N];
+      const cx<float> h0{0.5f * (P.re + Pm.re), 0.5f * (P.im - Pm.im)};
+      const cx<float> h16{0.5f * (P.im + Pm.im), 0.5f * (Pm.re - P.re)};
+      const int kyq = kfreq(ky, N), kzq = kfreq(kz, N);
+      const int qq = kyq * kyq + kzq * kzq;
+      const bool ny = ky == N / 2, nz = kz == N / 2;
+      const SymRow r0 = sym_row(tG[b][0][0], false, tG[b][1][ky], ny);
+      const SymRow r16 = sym_row(tG[b][0][N / 2], true, tG[b][1][ky], ny);
+      const cx<float> G0 = (wjp * sp[qq]) * sym_at(r0, tG[b][2][kz], nz);
+      const cx<float> G16 = (wjp * sp[qq + (N / 2) * (N / 2)]) * sym_at(r16, tG[b][2][kz], nz);
+      const cx<float> t0 = G0 * h0, t1 = G16 * h16;
+#if IB_FUSED_KAHAN
+      acc_add(acc2[h], cmp2[h], cx<float>{t0.re - t1.im, t0.im + t1.re});
+#else
+      acc2[h] = acc2[h] + cx<float>{t0.re - t1.im, t0.im + t1.re};
+#endif
+    }
+  };
 
-    cx<float> v[N];
-    // (1) E * fhat, inverse transform along kz (pencil (p, ky = l) in registers)
+  make_tables(0, 0);
+  __syncthreads();
+  make_x16(0);
+  __syncthreads();
+
+  float wj_prev = 0.f;
+  cx<float> v[N];
+  for (int mm = 0; mm < A.m; ++mm) {
+    const int b = mm & 1;
+    const float wj = (float)(crad_j * A.wsig[mm]);
+    // A: E * fhat, inverse along kz
     {
-      const bool ny = (l == N / 2);
+      const SymRow rr = sym_row(tE[b][0][p], false, tE[b][1][l], l == N / 2);
+      const cx<float>* fp = fh + (long long)p * PLANE + (long long)l * N;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        v[k] = sym_at(rr, tE[b][2][k], k == N / 2) * fp[k];
+        if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
       if (p == 0) {
-        const SymRow r0 = sym_row(tE[0][0], false, tE[1][l], ny);
-        const SymRow r16 = sym_row(tE[0][N / 2], true, tE[1][l], ny);
-        const cx<float>* f0 = fh + (long long)l * N;
-        const cx<float>* f16 = fh + 16 * PLANE + (long long)l * N;
 #pragma unroll
         for (int k = 0; k < N; ++k) {
-          const cx<float> a0 = sym_at(r0, tE[2][k], k == N / 2) * f0[k];
-          const cx<float> a1 = sym_at(r16, tE[2][k], k == N / 2) * f16[k];
-          v[k] = cx<float>{a0.re - a1.im, a0.im + a1.re};
-          if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-        }
-      } else {
-        const SymRow rr = sym_row(tE[0][p], false, tE[1][l], ny);
-        const cx<float>* fp = fh + (long long)p * PLANE + (long long)l * N;
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-          v[k] = sym_at(rr, tE[2][k], k == N / 2) * fp[k];
-          if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+          const cx<float> x = S16[l][k];
+          v[k] = cx<float>{v[k].re - x.im, v[k].im + x.re};
         }
       }
       fft_reg<N, +1>(v);
@@ -185,14 +230,15 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       for (int k = 0; k < N; ++k) W[p][l][k] = v[k];
     }
     __syncthreads();
-    // (2) inverse along ky (pencil (p, z = l))
+    // B: inverse along ky; packed plane of the previous pair
 #pragma unroll
     for (int k = 0; k < N; ++k) v[k] = W[p][k][l];
     fft_reg<N, +1>(v);
 #pragma unroll
     for (int k = 0; k < N; ++k) W[p][k][l] = v[k];
+    if (mm > 0) packed_acc(b ^ 1, wj_prev);
     __syncthreads();
-    // (3) along x: two real pencils (xy, xz), (xy, xz + 1) as one complex FFT
+    // C: along x, two real pencils (xy, xz), (xy, xz + 1) as one complex FFT
     {
       cx<float> Z[N];
       {
@@ -202,9 +248,9 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       }
 #pragma unroll
       for (int k = 1; k < N / 2; ++k) {
-        const cx<float> a = W[k][xy][xz], b = W[k][xy][xz + 1];
-        Z[k] = cx<float>{a.re - b.im, a.im + b.re};
-        Z[N - k] = cx<float>{a.re + b.im, b.re - a.im};
+        const cx<float> a = W[k][xy][xz], bb = W[k][xy][xz + 1];
+        Z[k] = cx<float>{a.re - bb.im, a.im + bb.re};
+        Z[N - k] = cx<float>{a.re + bb.im, bb.re - a.im};
       }
       fft_reg<N, +1>(Z);
       // 32-bit per-thread offset, made opaque per pair so the 32 addresses are
@@ -227,59 +273,52 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
         W[k][xy][xz + 1] = cx<float>{0.5f * (zk.im + zm.im), 0.5f * (zm.re - zk.re)};
       }
     }
+    if (mm + 1 < A.m) make_tables(mm + 1, b ^ 1);
     __syncthreads();
-    // (4) forward along ky
+    // D: forward along ky; X16 of the next pair
 #pragma unroll
     for (int k = 0; k < N; ++k) v[k] = W[p][k][l];
     fft_reg<N, -1>(v);
 #pragma unroll
     for (int k = 0; k < N; ++k) W[p][k][l] = v[k];
+    if (mm + 1 < A.m) make_x16(b ^ 1);
     __syncthreads();
-    // (5) forward along kz, then G * (.) and the compensated quadrature sum
+    // E: forward along kz, G * (.) and the sum over angles
 #pragma unroll
     for (int k = 0; k < N; ++k) v[k] = W[p][l][k];
     fft_reg<N, -1>(v);
     if (p == 0) {
 #pragma unroll
-      for (int k = 0; k < N; ++k) W[0][l][k] = v[k];
-    }
-    __syncthreads();
-    {
-      const bool ny = (l == N / 2);
-      if (p == 0) {
-        const int lm = (N - l) % N;
-        const SymRow r0 = sym_row(tG[0][0], false, tG[1][l], ny);
-        const SymRow r16 = sym_row(tG[0][N / 2], true, tG[1][l], ny);
+      for (int k = 0; k < N; ++k) SP[l][k] = v[k];
+    } else {
+      const SymRow rr = sym_row(tG[b][0][p], false, tG[b][1][l], l == N / 2);
+      const int q0 = p * p + kyf * kyf;
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-          const int kzf = kfreq(k, N);
-          const int qq = kyf * kyf + kzf * kzf;
-          const cx<float> pr = W[0][lm][(N - k) % N];
-          const cx<float> h0{0.5f * (v[k].re + pr.re), 0.5f * (v[k].im - pr.im)};
-          const cx<float> h16{0.5f * (v[k].im + pr.im), 0.5f * (pr.re - v[k].re)};
-          const cx<float> G0 = (wj * sp[qq]) * sym_at(r0, tG[2][k], k == N / 2);
-          const cx<float> G16 = (wj * sp[qq + (N / 2) * (N / 2)]) * sym_at(r16, tG[2][k], k == N / 2);
-          const cx<float> t0 = G0 * h0, t1 = G16 * h16;
-          acc_add(acc[k], cmp[IB_FUSED_KAHAN ? k : 0], cx<float>{t0.re - t1.im, t0.im + t1.re});
-        }
-      } else {
-        const SymRow rr = sym_row(tG[0][p], false, tG[1][l], ny);
-        const int q0 = p * p + kyf * kyf;
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-          const int kzf = kfreq(k, N);
-          const cx<float> G = (wj * sp[q0 + kzf * kzf]) * sym_at(rr, tG[2][k], k == N / 2);
-          acc_add(acc[k], cmp[IB_FUSED_KAHAN ? k : 0], G * v[k]);
-        }
+      for (int k = 0; k < N; ++k) {
+        const int kzf = kfreq(k, N);
+        const cx<float> G = (wj * sp[q0 + kzf * kzf]) * sym_at(rr, tG[b][2][k], k == N / 2);
+        acc_add(acc[k], cmp[IB_FUSED_KAHAN ? k : 0], G * v[k]);
       }
     }
+    wj_prev = wj;
   }
+  __syncthreads();
+  packed_acc((A.m - 1) & 1, wj_prev);
+
   cx<float>* out = A.part + ((A.cell0 + cell) * A.nr + j) * 2 * NPL * PLANE;
+  if (p != 0) {
 #pragma unroll
-  for (int k = 0; k < N; ++k) {
-    out[(long long)p * PLANE + (long long)l * N + k] = acc[k];
-    const cx<float> c = cmp[IB_FUSED_KAHAN ? k : 0];
-    out[NPL * PLANE + (long long)p * PLANE + (long long)l * N + k] = cx<float>{-c.re, -c.im};
+    for (int k = 0; k < N; ++k) {
+      out[(long long)p * PLANE + (long long)l * N + k] = acc[k];
+      const cx<float> c = cmp[IB_FUSED_KAHAN ? k : 0];
+      out[NPL * PLANE + (long long)p * PLANE + (long long)l * N + k] = cx<float>{-c.re, -c.im};
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int e = tid + h * THREADS;
+    out[e] = acc2[h];
+    out[NPL * PLANE + e] = cx<float>{-cmp2[h].re, -cmp2[h].im};
   }
 }
 
