@@ -1,0 +1,101 @@
+"""Autotuner of the collision engines' launch shapes.
+
+For a plan and a batch size it times the candidate settings of the engine the
+plan uses and keeps the fastest:
+  * fused FP32 form (3D 32^3): cells per fused launch (64 .. 512);
+  * staged form: the scratch budget, which sets the quadrature pairs per chunk
+    (items per pass launch) — 256 MiB .. 4 GiB.
+Results are cached in memory and in a JSON file (``INELASTIC_TUNE_CACHE`` or
+``~/.cache/inelastic_mi355x_tune.json``) keyed by (dim, n, nr, m, precision,
+batch bucket), so a process tunes each shape once.
+"""
+import json
+import os
+import pathlib
+
+_MEM = {}
+
+
+def _cache_path():
+    p = os.environ.get("INELASTIC_TUNE_CACHE")
+    return pathlib.Path(p) if p else pathlib.Path.home() / ".cache" / "inelastic_mi355x_tune.json"
+
+
+def _load_disk():
+    try:
+        return json.loads(_cache_path().read_text())
+    except Exception:
+        return {}
+
+
+def _store_disk(table):
+    try:
+        path = _cache_path()
+        path.parent.mkdir(parents=True, exist_ok=True)
+        path.write_text(json.dumps(table, indent=1, sort_keys=True))
+    except OSError:
+        pass
+
+
+def bucket(batch):
+    """Power-of-two batch bucket."""
+    b = 1
+    while b < batch:
+        b *= 2
+    return b
+
+
+def key_of(coll, batch):
+    r, _ = coll.sm.rquad()
+    sigma, _ = coll.sm.circ_or_sphr_quad()
+    return "d%d_n%d_nr%d_m%d_%s_b%d" % (coll.ndim, coll.sm.nv, len(r), sigma.shape[0],
+                                        coll.precision, bucket(batch))
+
+
+def candidates(coll, batch):
+    fused = coll.ndim == 3 and coll.sm.nv == 32 and coll.precision == "fp32"
+    if fused:
+        return [{"fused_cells": c} for c in (64, 128, 256, 512) if c <= max(64, batch)]
+    return [{"scratch_budget": b << 20} for b in (256, 1024, 4096)]
+
+
+def apply(coll, setting):
+    coll.set_tuning(chunk_pairs=0, fused_cells=setting.get("fused_cells", 0),
+                    scratch_budget=setting.get("scratch_budget", 0))
+
+
+def tune(coll, f_dev, with_loss=True, reps=2, force=False):
+    """Time every candidate on f_dev, apply and cache the fastest; returns it."""
+    from . import device as dev
+    t = dev.torch()
+    d = coll.ndim
+    batch = int(f_dev.numel() // (coll.sm.nv ** d))
+    k = key_of(coll, batch)
+    if not force:
+        if k in _MEM:
+            apply(coll, _MEM[k])
+            return _MEM[k]
+        disk = _load_disk()
+        if k in disk:
+            _MEM[k] = disk[k]["setting"]
+            apply(coll, _MEM[k])
+            return _MEM[k]
+    results = []
+    for cand in candidates(coll, batch):
+        apply(coll, cand)
+        coll.collide(f_dev, with_loss=with_loss)  # warm-up: sizes scratch
+        t.cuda.synchronize()
+        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            coll.collide(f_dev, with_loss=with_loss)
+        e1.record()
+        t.cuda.synchronize()
+        results.append((e0.elapsed_time(e1) / reps, cand))
+    ms, best = min(results, key=lambda x: x[0])
+    apply(coll, best)
+    _MEM[k] = best
+    disk = _load_disk()
+    disk[k] = {"setting": best, "ms": ms, "all": [(m, c) for m, c in results]}
+    _store_disk(disk)
+    return best

@@ -17,6 +17,8 @@ Extensions (keyword-only, all defaulting to the reference behaviour):
   gamma       per-cell VHS exponent array for ``collide`` (Maxwell and hard
               spheres mixed in one batched launch).
   project     apply the FP64 conservative projection (mass, momentum) to Q.
+  autotune    constructor keyword: time the engine's launch-shape candidates
+              on the first batch of each size bucket and keep the fastest.
 """
 import ctypes
 import math
@@ -32,10 +34,13 @@ class FSInelasticVHSCollision(BaseCollision):
 
     collision_model = "vhs"
 
-    def __init__(self, config, spectral_mesh, precision="fp64", with_loss=False, **kwargs):
+    def __init__(self, config, spectral_mesh, precision="fp64", with_loss=False, autotune=False,
+                 **kwargs):
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32'")
         self.precision = precision
+        self.autotune = bool(autotune)
+        self._tuned = set()
         self.call_with_loss = bool(with_loss)
         self._plan = None
         self._param_key = None
@@ -171,6 +176,12 @@ class FSInelasticVHSCollision(BaseCollision):
         lib = _lib.load()
 
         def go(f, out, nb):
+            if self.autotune:
+                from .. import autotune as at
+                b = at.bucket(nb)
+                if b not in self._tuned:
+                    at.tune(self, f, with_loss=with_loss)
+                    self._tuned.add(b)
             pidx = self._param_index(nb, e, e0, gamma)
             _lib.check(lib.ib_collide(self._plan, f.data_ptr(), out.data_ptr(), nb,
                                       dev.ptr(pidx), int(bool(with_loss)),
