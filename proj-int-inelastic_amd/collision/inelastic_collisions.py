@@ -180,8 +180,8 @@ class FSInelasticVHSCollision(BaseCollision):
                 from .. import autotune as at
                 b = at.bucket(nb)
                 if b not in self._tuned:
+                    self._tuned.add(b)  # before timing: tune() calls collide()
                     at.tune(self, f, with_loss=with_loss)
-                    self._tuned.add(b)
             pidx = self._param_index(nb, e, e0, gamma)
             _lib.check(lib.ib_collide(self._plan, f.data_ptr(), out.data_ptr(), nb,
                                       dev.ptr(pidx), int(bool(with_loss)),

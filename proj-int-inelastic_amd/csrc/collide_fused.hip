@@ -36,10 +36,12 @@
 namespace ib {
 
 // FP64 staged-engine pieces (collide_plain.hip)
-int staged_prologue_fp64_332(Plan* P, const double* f, long long nb, void* fhat64, void* work);
-int staged_epilogue_fp64_332(Plan* P, const double* f, double* q, long long nb, void* acc64,
-                             void* fhat64, void* work, int with_loss, double hb, long long cell0,
-                             const int* pidx);
+int staged_prologue_fp64(Plan* P, const double* f, long long nb, void* fhat64, void* work);
+int staged_epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void* acc64,
+                         void* fhat64, void* work, int with_loss, double hb, const int* pidx);
+// fused 2D FP32 engine (collide_fused2.hip)
+int collide_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                   int with_loss, double hb, hipStream_t st, int* handled);
 
 namespace f3 {
 constexpr int N = 32;
@@ -380,6 +382,8 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
                   int with_loss, double hb, hipStream_t st, int* handled) {
   using namespace f3;
   *handled = 0;
+  if (P->precision == IB_PREC_FP32 && P->dim == 2)
+    return collide_fused2(P, f, q, batch, pidx, with_loss, hb, st, handled);
   if (!(P->dim == 3 && P->n == 32 && P->precision == IB_PREC_FP32)) return IB_OK;
   *handled = 1;
   const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 256);
@@ -397,7 +401,7 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
   for (long long c0 = 0; c0 < batch; c0 += sub) {
     const long long nb = std::min(sub, batch - c0);
     const double* fc = f + c0 * NPTS;
-    if ((rc = staged_prologue_fp64_332(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
+    if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
     const long long tot = nb * NPTS;
     hipLaunchKernelGGL(k_f3_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                        (const cx<double>*)P->fhat.ptr, fc, fh32, f32, nb);
@@ -429,8 +433,8 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
     hipLaunchKernelGGL(k_f3_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
                        (cx<double>*)P->acc.ptr, nb, P->nr);
     IB_HIP(hipGetLastError());
-    if ((rc = staged_epilogue_fp64_332(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr,
-                                       P->work.ptr, with_loss, hb, 0, pidx ? pidx + c0 : nullptr)))
+    if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
+                                   with_loss, hb, pidx ? pidx + c0 : nullptr)))
       return rc;
   }
   return IB_OK;

@@ -1,0 +1,239 @@
+// Fused FP32 form of Q(f, f) for 2D grids (N = 8, 16, 32).
+//
+// Same operator as reference src/collision/inelastic_collisions.py:208-227 with
+// C2C transforms (the reference's own semantics: the N x N complex work array
+// of a quadrature pair fits in LDS, so no real-data reduction is needed).  A
+// workgroup serves one cell and a strided subset of its (j, m) quadrature
+// pairs with G = THREADS / N concurrent pairs, one N-thread group per pair:
+//   A  row pencils: E * fhat, inverse along axis 1
+//   B  column pencils: inverse along axis 0, * f, forward along axis 0
+//   C  row pencils: forward along axis 1, * G, sum over the group's pairs in
+//      registers
+// fhat / n and f of the cell are staged in LDS once per workgroup.  The
+// groups' sums are added in LDS, each workgroup writes an FP32 partial; the
+// FP64 sum over workgroups, the final inverse transform, the loss term and the
+// Laplacian run in FP64 (collide_plain.hip), as does the prologue fhat = fft(f).
+#include <algorithm>
+
+#include "common.h"
+#include "plan.h"
+
+namespace ib {
+
+int staged_prologue_fp64(Plan* P, const double* f, long long nb, void* fhat64, void* work);
+int staged_epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void* acc64,
+                         void* fhat64, void* work, int with_loss, double hb, const int* pidx);
+
+struct F2Args {
+  const cx<float>* fh;  // [cell][N][N], fhat / n
+  const float* f32;     // [cell][N][N]
+  cx<float>* part;      // [cell][wpc][N][N]
+  const int* pidx;
+  long long batch;
+  int wpc;
+  const double* sigma;
+  const double* rj;
+  const double* wsig;
+  const double* crad;
+  const double* a;
+  const float* spg;
+  int nr, m, nq;
+  double L;
+};
+
+template <int N>
+struct F2Shape {
+  static constexpr int THREADS = N >= 32 ? 128 : 256;
+  static constexpr int G = THREADS / N;
+};
+
+template <int N>
+__global__ __launch_bounds__(F2Shape<N>::THREADS) void k_fused2(F2Args A) {
+  constexpr int THREADS = F2Shape<N>::THREADS, G = F2Shape<N>::G, PAD = N + 1;
+  __shared__ cx<float> Wg[G][N][PAD];
+  __shared__ cx<float> fhs[N][PAD];
+  __shared__ float fs[N][PAD];
+  __shared__ cx<float> tab[G][4][N];  // E axis 0, E axis 1, G axis 0, G axis 1
+  const int tid = threadIdx.x;
+  const int g = tid / N, l = tid % N;
+  const long long cell = blockIdx.x / A.wpc;
+  const int w = (int)(blockIdx.x % A.wpc);
+  if (cell >= A.batch) return;  // uniform over the workgroup
+  const int ps = A.pidx ? A.pidx[cell] : 0;
+  for (int e = tid; e < N * N; e += THREADS) {
+    fhs[e / N][e % N] = A.fh[cell * N * N + e];
+    fs[e / N][e % N] = A.f32[cell * N * N + e];
+  }
+  const int npairs = A.nr * A.m;
+  const int stride = A.wpc * G;
+  const int rounds = npairs > w * G ? (npairs - w * G + stride - 1) / stride : 0;
+  const int kl = kfreq(l, N);
+  cx<float> acc[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = cx<float>{0.f, 0.f};
+  __syncthreads();
+  cx<float> v[N];
+  for (int r = 0; r < rounds; ++r) {
+    const int pair = w * G + g + r * stride;
+    const bool act = pair < npairs;
+    const int j = act ? pair / A.m : 0, mm = act ? pair % A.m : 0;
+    if (act) {
+      const double rj = A.rj[j];
+      const double aj = A.a[ps * A.nr + j] * rj;
+      const double kf = (double)kl;
+#pragma unroll
+      for (int ax = 0; ax < 2; ++ax) {
+        const double s = A.sigma[mm * 2 + ax];
+        double sn, cs;
+        sincos(-(M_PI * rj * s * kf) / A.L, &sn, &cs);
+        tab[g][ax][l] = cx<float>{(float)cs, (float)sn};
+        sincos(aj * s * kf, &sn, &cs);
+        tab[g][2 + ax][l] = cx<float>{(float)cs, (float)sn};
+      }
+    }
+    __syncthreads();
+    if (act) {  // A
+      const cx<float> e0 = tab[g][0][l];
+#pragma unroll
+      for (int k = 0; k < N; ++k) v[k] = (e0 * tab[g][1][k]) * fhs[l][k];
+      fft_reg<N, +1>(v);
+#pragma unroll
+      for (int k = 0; k < N; ++k) Wg[g][l][k] = v[k];
+    }
+    __syncthreads();
+    if (act) {  // B
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = Wg[g][i][l];
+      fft_reg<N, +1>(v);
+#pragma unroll
+      for (int i = 0; i < N; ++i) v[i] = fs[i][l] * v[i];
+      fft_reg<N, -1>(v);
+#pragma unroll
+      for (int i = 0; i < N; ++i) Wg[g][i][l] = v[i];
+    }
+    __syncthreads();
+    if (act) {  // C
+#pragma unroll
+      for (int k = 0; k < N; ++k) v[k] = Wg[g][l][k];
+      fft_reg<N, -1>(v);
+      const float wj = (float)(A.crad[ps * A.nr + j] * A.wsig[mm]);
+      const float* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
+      const cx<float> g0 = tab[g][2][l];
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const int kk = kfreq(k, N);
+        const cx<float> Gk = (wj * sp[kl * kl + kk * kk]) * (g0 * tab[g][3][k]);
+        acc[k] = acc[k] + Gk * v[k];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) Wg[g][l][k] = acc[k];
+  __syncthreads();
+  cx<float>* out = A.part + (cell * A.wpc + w) * N * N;
+  for (int e = tid; e < N * N; e += THREADS) {
+    cx<float> s{0.f, 0.f};
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) s = s + Wg[gg][e / N][e % N];
+    out[e] = s;
+  }
+}
+
+__global__ void k_f2_pack(const cx<double>* __restrict__ fhat, const double* __restrict__ f,
+                          cx<float>* __restrict__ fh32, float* __restrict__ f32, long long total,
+                          double inv_n) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  f32[gid] = (float)f[gid];
+  const cx<double> v = fhat[gid];
+  fh32[gid] = cx<float>{(float)(v.re * inv_n), (float)(v.im * inv_n)};
+}
+
+__global__ void k_f2_reduce(const cx<float>* __restrict__ part, cx<double>* __restrict__ acc64,
+                            long long nb, int wpc, long long npts) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nb * npts) return;
+  const long long c = gid / npts, e = gid % npts;
+  double re = 0.0, im = 0.0;
+  for (int w = 0; w < wpc; ++w) {
+    const cx<float> x = part[(c * wpc + w) * npts + e];
+    re += (double)x.re;
+    im += (double)x.im;
+  }
+  acc64[gid] = cx<double>{re, im};
+}
+
+template <int N>
+static int run_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                      int with_loss, double hb, hipStream_t st) {
+  constexpr int G = F2Shape<N>::G, THREADS = F2Shape<N>::THREADS;
+  constexpr long long NPTS = (long long)N * N;
+  const int npairs = P->nr * P->m;
+  const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 4096);
+  int wpc = (int)std::max<long long>(1, (2048 + sub - 1) / sub);
+  wpc = std::min(wpc, std::max(1, (npairs + G - 1) / G));
+  const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
+  int rc;
+  if ((rc = P->fhat.ensure(sub * b64))) return rc;
+  if ((rc = P->acc.ensure(sub * b64))) return rc;
+  if ((rc = P->work.ensure(2 * sub * b64))) return rc;
+  const size_t part_b = (size_t)sub * wpc * NPTS * sizeof(cx<float>);
+  const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<float>);
+  if ((rc = P->aux.ensure(part_b + fh_b + (size_t)sub * NPTS * sizeof(float)))) return rc;
+  cx<float>* part = (cx<float>*)P->aux.ptr;
+  cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + part_b);
+  float* f32 = (float*)((char*)fh32 + fh_b);
+  for (long long c0 = 0; c0 < batch; c0 += sub) {
+    const long long nb = std::min(sub, batch - c0);
+    const double* fc = f + c0 * NPTS;
+    const int* pc = pidx ? pidx + c0 : nullptr;
+    if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
+    const long long tot = nb * NPTS;
+    hipLaunchKernelGGL(k_f2_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       (const cx<double>*)P->fhat.ptr, fc, fh32, f32, tot, 1.0 / (double)NPTS);
+    IB_HIP(hipGetLastError());
+    F2Args A{};
+    A.fh = fh32;
+    A.f32 = f32;
+    A.part = part;
+    A.pidx = pc;
+    A.batch = nb;
+    A.wpc = wpc;
+    A.sigma = P->d_sigma;
+    A.rj = P->d_rj;
+    A.wsig = P->d_wsig;
+    A.crad = P->d_crad;
+    A.a = P->d_a;
+    A.spg = (const float*)P->d_spg;
+    A.nr = P->nr;
+    A.m = P->m;
+    A.nq = P->nq;
+    A.L = P->L;
+    if ((rc = P->timed_begin(st))) return rc;
+    hipLaunchKernelGGL(k_fused2<N>, dim3((unsigned)(nb * wpc)), dim3(THREADS), 0, st, A);
+    IB_HIP(hipGetLastError());
+    P->timed_end(st, nb);
+    hipLaunchKernelGGL(k_f2_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
+                       (cx<double>*)P->acc.ptr, nb, wpc, NPTS);
+    IB_HIP(hipGetLastError());
+    if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
+                                   with_loss, hb, pc)))
+      return rc;
+  }
+  return IB_OK;
+}
+
+int collide_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                   int with_loss, double hb, hipStream_t st, int* handled) {
+  *handled = 1;
+  switch (P->n) {
+    case 16: return run_fused2<16>(P, f, q, batch, pidx, with_loss, hb, st);
+    case 32: return run_fused2<32>(P, f, q, batch, pidx, with_loss, hb, st);
+    default:
+      *handled = 0;
+      return IB_OK;
+  }
+}
+
+}  // namespace ib

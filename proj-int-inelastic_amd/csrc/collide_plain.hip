@@ -803,26 +803,27 @@ int collide_group(Plan** P, int np, const double* const* f, double* const* q,
 }
 
 // ---------------------------------------------------------------------------
-// FP64 prologue / epilogue of the fused FP32 engine on 3D 32^3 grids.
-int staged_prologue_fp64_332(Plan* P, const double* f, long long nb, void* fhat64, void* work) {
+// FP64 prologue / epilogue of the fused FP32 engines (any grid shape).
+template <int N, int D>
+static int prologue_fp64(Plan* P, const double* f, long long nb, void* fhat64, void* work) {
   hipStream_t st = P->stream;
   BArgs<double> B{};
   B.f = f;
   B.w = (cx<double>*)work;
   B.items = nb;
   B.P = 1;
-  IB_TRY((launch_b<double, 32, 3, B_PRO>(B, st)));
+  IB_TRY((launch_b<double, N, D, B_PRO>(B, st)));
   AArgs<double> A{};
   A.src = (const cx<double>*)work;
   A.dst = (cx<double>*)fhat64;
   A.items = nb;
-  IB_TRY((launch_a<double, 32, 3, A_OUT_PRO>(A, st)));
+  IB_TRY((launch_a<double, N, D, A_OUT_PRO>(A, st)));
   return IB_OK;
 }
 
-int staged_epilogue_fp64_332(Plan* P, const double* f, double* q, long long nb, void* acc64,
-                             void* fhat64, void* work, int with_loss, double hb, long long cell0,
-                             const int* pidx) {
+template <int N, int D>
+static int epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void* acc64,
+                         void* fhat64, void* work, int with_loss, double hb, const int* pidx) {
   hipStream_t st = P->stream;
   AArgs<double> A{};
   A.src = (const cx<double>*)acc64;
@@ -835,8 +836,7 @@ int staged_epilogue_fp64_332(Plan* P, const double* f, double* q, long long nb, 
   A.L = P->L;
   A.hb = hb;
   A.with_loss = with_loss;
-  A.cell0 = cell0;
-  IB_TRY((launch_a<double, 32, 3, A_IN_EPI>(A, st)));
+  IB_TRY((launch_a<double, N, D, A_IN_EPI>(A, st)));
   BArgs<double> B{};
   B.w = (cx<double>*)work;
   B.f = f;
@@ -844,8 +844,31 @@ int staged_epilogue_fp64_332(Plan* P, const double* f, double* q, long long nb, 
   B.items = nb;
   B.P = 1;
   B.with_loss = with_loss;
-  IB_TRY((launch_b<double, 32, 3, B_EPI>(B, st)));
+  IB_TRY((launch_b<double, N, D, B_EPI>(B, st)));
   return IB_OK;
+}
+
+int staged_prologue_fp64(Plan* P, const double* f, long long nb, void* fhat64, void* work) {
+  switch (P->dim * 100 + P->n) {
+    case 216: return prologue_fp64<16, 2>(P, f, nb, fhat64, work);
+    case 232: return prologue_fp64<32, 2>(P, f, nb, fhat64, work);
+    case 332: return prologue_fp64<32, 3>(P, f, nb, fhat64, work);
+    default:
+      set_error("no FP64 prologue for this shape");
+      return IB_ERR_UNSUPPORTED;
+  }
+}
+
+int staged_epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void* acc64,
+                         void* fhat64, void* work, int with_loss, double hb, const int* pidx) {
+  switch (P->dim * 100 + P->n) {
+    case 216: return epilogue_fp64<16, 2>(P, f, q, nb, acc64, fhat64, work, with_loss, hb, pidx);
+    case 232: return epilogue_fp64<32, 2>(P, f, q, nb, acc64, fhat64, work, with_loss, hb, pidx);
+    case 332: return epilogue_fp64<32, 3>(P, f, q, nb, acc64, fhat64, work, with_loss, hb, pidx);
+    default:
+      set_error("no FP64 epilogue for this shape");
+      return IB_ERR_UNSUPPORTED;
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -241,6 +241,33 @@ def test_fused_fp32_3d32_batch(pkg, need_gpu, with_loss, hb):
     assert worst <= TOL32, worst
 
 
+@pytest.mark.parametrize("N,nr,M,s", [(16, 8, 8, 2.0), (32, 32, 8, 2.718491035932)])
+@pytest.mark.parametrize("with_loss,hb", [(True, 0.0), (False, 0.05)])
+def test_fused_fp32_2d_batch(pkg, need_gpu, N, nr, M, s, with_loss, hb):
+    """Fused FP32 form of 2D grids (C2C, one N-thread group per quadrature pair):
+    13 cells with per-cell e and gamma, in sub-batches of 5 cells, against the
+    FP64 oracle cell by cell."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    d = 2
+    cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 0.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision="fp32")
+    coll.set_tuning(fused_cells=5)
+    es = np.linspace(0.3, 1.0, 13)
+    gs = np.array([0.0, 1.0] * 6 + [0.0])
+    f = _data(d, N, sm.L, 13, seed=N)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=with_loss, heat_bath=hb, e=es,
+                     gamma=gs).cpu().numpy()
+    worst = 0.0
+    for c in range(13):
+        op = _oracle(d, N, nr, M, s, es[c], es[c], gs[c])
+        ref = op(f[c], heat_bath=hb) if not with_loss else op.collide(f[c], with_loss=True)
+        worst = max(worst, _err(q[c], ref))
+    print("fused FP32 2D", N, "worst rel Linf", worst)
+    assert worst <= TOL32, worst
+
+
 def test_fused_fp32_smooth_data_vs_c2c(pkg, need_gpu):
     """On grid-resolved data the FP32 form meets 1e-5 against the reference's C2C operator."""
     import torch
@@ -316,7 +343,8 @@ def test_autotune_picks_a_candidate_and_keeps_results(pkg, need_gpu, tmp_path, m
     from proj_int_inelastic_amd import autotune
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
     monkeypatch.setenv("INELASTIC_TUNE_CACHE", str(tmp_path / "tune.json"))
-    for d, N, nr, M, s, prec in [(2, 16, 8, 8, 2.5, "fp64"), (3, 32, 4, 6, 3.624654714576, "fp32")]:
+    for d, N, nr, M, s, prec in [(2, 16, 8, 8, 2.5, "fp64"), (2, 32, 8, 8, 3.0, "fp32"),
+                                 (3, 32, 4, 6, 3.624654714576, "fp32")]:
         cfg = _cfg(d, N, nr, M, s, 0.8, 0.8, 1.0)
         sm = SpectralMesh(cfg)
         plain = FSInelasticVHSCollision(cfg, sm, precision=prec)
