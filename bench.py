@@ -311,7 +311,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 (complex FFTs, Kahan quadrature sums) + fp64 moments/projection",
+        "dtype": "fp32 (FFTs, angular quadrature sums) + fp64 (radial sum, final transform, loss, projection)",
         "data": "synthetic (seeded anisotropic Gaussians, e ~ U[0.5,1])",
         "config": {"workload": "3D velocity batch: 1024 cells x 32^3, Nr=32, M=38 Lebedev, "
                                "hard spheres, per-cell e, gain+loss+projection",
