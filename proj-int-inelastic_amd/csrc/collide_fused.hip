@@ -14,11 +14,13 @@
 // loops over the M angles.  The 16 packed kx-planes (32 x 33 padded complex
 // FP32 = 132 KiB) stay in LDS for the whole pair; the pass along kz is done
 // with the pencil in registers and fused with E * fhat on the way in and with
-// G * (.) plus the Kahan-compensated quadrature sum on the way out; the
+// G * (.) plus the sum over angles on the way out; the
 // pass along x turns two real pencils into one complex FFT.  Workgroups of
 // one cell are numbered so they land on the same XCD (the dispatcher assigns
 // workgroup i to XCD i mod 8), so fhat and f of a cell are read from one L2.
-// Per-(cell, j) Kahan sums (hi, lo) go to a partial buffer; the FP64 reduce
+// sp(a_j r_j |k|) and the radial weight do not depend on the angle, so the
+// sum over angles runs on phase * transform only and is scaled once at the
+// end.  Per-(cell, j) sums go to a partial buffer; the FP64 reduce
 // over j, the final inverse transform, the loss term and the Laplacian run in
 // FP64 (collide_plain.hip kernels), as does the prologue fhat = fft(f).
 #include <algorithm>
@@ -50,12 +52,13 @@ constexpr int PAD = N + 1;
 constexpr int THREADS = 512;
 constexpr long long NPTS = (long long)N * N * N;
 constexpr long long PLANE = (long long)N * N;
+constexpr int KP = IB_FUSED_KAHAN ? 2 : 1;  // partial planes per (cell, j): sum (, compensation)
 }  // namespace f3
 
 struct F3Args {
   const cx<float>* fh;  // [cell][17][32][32], kx = 0..16, scaled by 1/n
   const float* f32;     // [cell][32][32][32]
-  cx<float>* part;      // [cell][nr][2][16][32][32] (hi, lo)
+  cx<float>* part;      // [cell][nr][KP][16][32][32] (sum, and compensation when KP = 2)
   const int* pidx;
   long long batch;
   long long cell0;
@@ -211,6 +214,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   for (int mm = 0; mm < A.m; ++mm) {
     const int b = mm & 1;
     const float wj = (float)(crad_j * A.wsig[mm]);
+    const float ws_half = (float)(0.5 * A.wsig[mm]);
     // A: E * fhat, inverse along kz
     {
       const SymRow rr = sym_row(tE[b][0][p], false, tE[b][1][l], l == N / 2);
@@ -271,8 +275,9 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
 #pragma unroll
       for (int k = 1; k < N / 2; ++k) {
         const cx<float> zk = Z[k], zm = Z[N - k];
-        W[k][xy][xz] = cx<float>{0.5f * (zk.re + zm.re), 0.5f * (zk.im - zm.im)};
-        W[k][xy][xz + 1] = cx<float>{0.5f * (zk.im + zm.im), 0.5f * (zm.re - zk.re)};
+        // planes 1..15 carry twice their value; phase E folds the 1/2 into its weight
+        W[k][xy][xz] = cx<float>{zk.re + zm.re, zk.im - zm.im};
+        W[k][xy][xz + 1] = cx<float>{zk.im + zm.im, zm.re - zk.re};
       }
     }
     if (mm + 1 < A.m) make_tables(mm + 1, b ^ 1);
@@ -293,34 +298,40 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
 #pragma unroll
       for (int k = 0; k < N; ++k) SP[l][k] = v[k];
     } else {
-      const SymRow rr = sym_row(tG[b][0][p], false, tG[b][1][l], l == N / 2);
-      const int q0 = p * p + kyf * kyf;
+      SymRow rr = sym_row(tG[b][0][p], false, tG[b][1][l], l == N / 2);
+      rr.rest = ws_half * rr.rest;
+      rr.row = ws_half * rr.row;
 #pragma unroll
-      for (int k = 0; k < N; ++k) {
-        const int kzf = kfreq(k, N);
-        const cx<float> G = (wj * sp[q0 + kzf * kzf]) * sym_at(rr, tG[b][2][k], k == N / 2);
-        acc_add(acc[k], cmp[IB_FUSED_KAHAN ? k : 0], G * v[k]);
-      }
+      for (int k = 0; k < N; ++k)
+        acc_add(acc[k], cmp[IB_FUSED_KAHAN ? k : 0], sym_at(rr, tG[b][2][k], k == N / 2) * v[k]);
     }
     wj_prev = wj;
   }
   __syncthreads();
   packed_acc((A.m - 1) & 1, wj_prev);
 
-  cx<float>* out = A.part + ((A.cell0 + cell) * A.nr + j) * 2 * NPL * PLANE;
+  cx<float>* out = A.part + ((A.cell0 + cell) * A.nr + j) * KP * NPL * PLANE;
   if (p != 0) {
+    const float cj = (float)crad_j;
+    const int q0 = p * p + kyf * kyf;
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      out[(long long)p * PLANE + (long long)l * N + k] = acc[k];
-      const cx<float> c = cmp[IB_FUSED_KAHAN ? k : 0];
-      out[NPL * PLANE + (long long)p * PLANE + (long long)l * N + k] = cx<float>{-c.re, -c.im};
+      const int kzf = kfreq(k, N);
+      const float g = cj * sp[q0 + kzf * kzf];
+      out[(long long)p * PLANE + (long long)l * N + k] = g * acc[k];
+#if IB_FUSED_KAHAN
+      const cx<float> c = cmp[k];
+      out[NPL * PLANE + (long long)p * PLANE + (long long)l * N + k] = -g * c;
+#endif
     }
   }
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int e = tid + h * THREADS;
     out[e] = acc2[h];
+#if IB_FUSED_KAHAN
     out[NPL * PLANE + e] = cx<float>{-cmp2[h].re, -cmp2[h].im};
+#endif
   }
 }
 
@@ -341,7 +352,7 @@ __global__ void k_f3_pack(const cx<double>* __restrict__ fhat, const double* __r
   }
 }
 
-// Sum the per-j Kahan partials in FP64 and expand the packed half spectrum to
+// Sum the per-j partials in FP64 and expand the packed half spectrum to
 // the full FP64 spectrum (Hermitian completion), as the FP64 epilogue expects.
 __global__ void k_f3_reduce(const cx<float>* __restrict__ part, cx<double>* __restrict__ acc64,
                             long long nb, int nr) {
@@ -352,12 +363,16 @@ __global__ void k_f3_reduce(const cx<float>* __restrict__ part, cx<double>* __re
   const int kx = (int)(i / PLANE), ky = (int)((i / N) % N), kz = (int)(i % N);
   auto sum_at = [&](int pl, int yy, int zz) {
     double re = 0.0, im = 0.0;
-    const cx<float>* b = part + c * nr * 2 * NPL * PLANE + (long long)pl * PLANE + (long long)yy * N + zz;
+    const cx<float>* b = part + c * nr * KP * NPL * PLANE + (long long)pl * PLANE + (long long)yy * N + zz;
     for (int j = 0; j < nr; ++j) {
-      const cx<float> hi = b[(long long)j * 2 * NPL * PLANE];
-      const cx<float> lo = b[(long long)j * 2 * NPL * PLANE + NPL * PLANE];
-      re += (double)hi.re + (double)lo.re;
-      im += (double)hi.im + (double)lo.im;
+      const cx<float> hi = b[(long long)j * KP * NPL * PLANE];
+      re += (double)hi.re;
+      im += (double)hi.im;
+      if (KP == 2) {
+        const cx<float> lo = b[(long long)j * KP * NPL * PLANE + NPL * PLANE];
+        re += (double)lo.re;
+        im += (double)lo.im;
+      }
     }
     return cx<double>{re, im};
   };
@@ -388,7 +403,7 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
   *handled = 1;
   const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 256);
   const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
-  const size_t part_bytes = (size_t)P->nr * 2 * NPL * PLANE * sizeof(cx<float>);
+  const size_t part_bytes = (size_t)P->nr * KP * NPL * PLANE * sizeof(cx<float>);
   const size_t fh32_bytes = (size_t)17 * PLANE * sizeof(cx<float>) + (size_t)NPTS * sizeof(float);
   int rc;
   if ((rc = P->fhat.ensure(sub * b64))) return rc;
