@@ -211,6 +211,12 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
 
   float wj_prev = 0.f;
   cx<float> v[N];
+  // fhat along this thread's kz pencil is the same for every pair: phase E
+  // reloads it into the pencil registers as they free up, so the L2 latency
+  // overlaps the sum over angles
+  const cx<float>* fp = fh + (long long)p * PLANE + (long long)l * N;
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = fp[k];
   for (int mm = 0; mm < A.m; ++mm) {
     const int b = mm & 1;
     const float wj = (float)(crad_j * A.wsig[mm]);
@@ -218,12 +224,8 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     // A: E * fhat, inverse along kz
     {
       const SymRow rr = sym_row(tE[b][0][p], false, tE[b][1][l], l == N / 2);
-      const cx<float>* fp = fh + (long long)p * PLANE + (long long)l * N;
 #pragma unroll
-      for (int k = 0; k < N; ++k) {
-        v[k] = sym_at(rr, tE[b][2][k], k == N / 2) * fp[k];
-        if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-      }
+      for (int k = 0; k < N; ++k) v[k] = sym_at(rr, tE[b][2][k], k == N / 2) * v[k];
       if (p == 0) {
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -296,14 +298,19 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     fft_reg<N, -1>(v);
     if (p == 0) {
 #pragma unroll
-      for (int k = 0; k < N; ++k) SP[l][k] = v[k];
+      for (int k = 0; k < N; ++k) {
+        SP[l][k] = v[k];
+        v[k] = fp[k];
+      }
     } else {
       SymRow rr = sym_row(tG[b][0][p], false, tG[b][1][l], l == N / 2);
       rr.rest = ws_half * rr.rest;
       rr.row = ws_half * rr.row;
 #pragma unroll
-      for (int k = 0; k < N; ++k)
+      for (int k = 0; k < N; ++k) {
         acc_add(acc[k], cmp[IB_FUSED_KAHAN ? k : 0], sym_at(rr, tG[b][2][k], k == N / 2) * v[k]);
+        v[k] = fp[k];
+      }
     }
     wj_prev = wj;
   }
