@@ -120,6 +120,26 @@ int ib_tvd1d(const double* f, double* df, int nx, long long nv, const double* v1
              double dtdx, int order, int limiter, int bc_lower, int bc_upper, int vx_rows,
              void* stream);
 
+/* Flux-form second-order upwind scheme with the van Leer limiter along axis 0
+ * of (n, inner) float64 arrays (the stencil's ghost rows included, as the
+ * reference's callers pass them).  mode 0: grad_vf1 (out (n-2, inner));
+ * 1: grad_vf2 (out (n-4, inner)); v is (n, inner) for modes 0 and 1.
+ * mode 2: F_p_2, mode 3: F_m_2 (out (n-4, inner)) with v the speed aligned with
+ * f[1:-1], (n-2, inner).  Same evaluation order as numpy.
+ * Replaces src/riemann/upwind.py:40-50 (grad_vf2, grad_vf1) and :20-37 (F_p_2, F_m_2). */
+int ib_upwind_flux(const double* f, const double* v, double* out, long long n, long long inner,
+                   double dx, double dt, int mode, void* stream);
+
+/* One fast-sweeping iteration, in place on f (nx, nvr, nvc): for velocity rows
+ * with vx > 0, f[i] = (nu gain[i] + (vx/dx) f[i-1]) / (nu rho[i] + vx/dx) for
+ * i = 1..nx-2 in order; rows with vx < 0 sweep i = nx-2..1 from f[i+1]; rows with
+ * vx = 0 and the end cells are kept.  gain (nx, nvr, nvc) is the gain term Q+(f)
+ * of the iterate, rho (nx,) its density, vx (nvr,) the row velocities.  *err
+ * (device) receives max |f_new - f_old|.
+ * Replaces src/iter_solver.py:38-62 (BoltzmannFastSweep.step after its Q+ call). */
+int ib_fast_sweep(double* f, const double* gain, const double* rho, const double* vx, int nx,
+                  int nvr, long long nvc, double dx, double nu, double* err, void* stream);
+
 /* y <- y + a x over n float64 values.  Replaces the `state.f += M * df` updates of
  * src/boltzmann.py:84,126-128,137-140,153-158. */
 int ib_axpy(double* y, const double* x, double a, long long n, void* stream);

@@ -12,10 +12,11 @@ Follows the reference:
 
 New: the MC limiter phi = max(0, min((1+theta)/2, 2, 2 theta)).
 
-Pins (tests/test_oracle_transport.py): exact transport of piecewise constant
-data at CFL 1, discrete conservation with periodic ghosts, total variation
-non-increase, and golden vectors written from the reference's own
-numpy-only modules (tests/golden/).
+Pins (tests/test_oracle_transport_integrators.py, tests/test_golden_transport.py):
+exact transport of piecewise constant data at CFL 1, discrete conservation
+with periodic ghosts, total variation non-increase, and golden vectors written
+from the reference's own numpy-only modules (tests/golden/) for the TVD step,
+the limiter and the flux-form scheme.
 """
 import numpy as np
 
@@ -156,4 +157,5 @@ def grad_vf1(f, v):
     """upwind.py:47-50."""
     vp = np.maximum(v, 0)
     vm = np.minimum(v, 0)
-    return (vp * f)[1:-1] - (vp * f)[:-2] + (vm * f)[2:] - (vm * f)[1:-1]
+    up, dn = vp * f, vm * f
+    return (up[1:-1] - up[:-2]) + (dn[2:] - dn[1:-1])

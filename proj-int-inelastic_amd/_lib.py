@@ -57,6 +57,8 @@ SIGNATURES = {
     "ib_moments": (_I, [_P, _P, _LL, _I, _I, _D, _P]),
     "ib_project_conservative": (_I, [_P, _LL, _I, _I, _D, _P]),
     "ib_tvd1d": (_I, [_P, _P, _I, _LL, _P, _D, _I, _I, _I, _I, _I, _P]),
+    "ib_upwind_flux": (_I, [_P, _P, _P, _LL, _LL, _D, _D, _I, _P]),
+    "ib_fast_sweep": (_I, [_P, _P, _P, _P, _I, _I, _LL, _D, _D, _P, _P]),
     "ib_axpy": (_I, [_P, _P, _D, _LL, _P]),
     "ib_df_combine": (_I, [_P, _P, _P, _D, _D, _LL, _P]),
     "ib_host_sp": (_D, [_I, _D]),

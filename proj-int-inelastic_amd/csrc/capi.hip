@@ -72,6 +72,10 @@ int launch_df_combine(const double* adv, const double* q, double* out, double dt
 int launch_moments(const double* f, double* out, long long batch, int dim, int n, double L,
                    hipStream_t st);
 int launch_project(double* q, long long batch, int dim, int n, double L, hipStream_t st);
+int launch_upwind(const double* f, const double* v, double* out, long long n, long long inner,
+                  double dx, double dt, int mode, hipStream_t st);
+int launch_fast_sweep(double* f, const double* gain, const double* rho, const double* vx, int nx,
+                      int nvr, long long nvc, double dx, double nu, double* err, hipStream_t st);
 
 template <typename X>
 static int upload(X** dst, const X* src, size_t count) {
@@ -398,6 +402,29 @@ int ib_tvd1d(const double* f, double* df, int nx, long long nv, const double* v1
   }
   return launch_tvd1d(f, df, nx, nv, v1, dtdx, order, limiter, bc_lower, bc_upper,
                       (hipStream_t)stream);
+}
+
+int ib_upwind_flux(const double* f, const double* v, double* out, long long n, long long inner,
+                   double dx, double dt, int mode, void* stream) {
+  if (mode < 0 || mode > 3 || inner < 0 || !f || !v || !out) {
+    set_error("invalid upwind_flux arguments");
+    return IB_ERR_ARG;
+  }
+  if (n < (mode == 0 ? 2 : 4)) {
+    set_error(mode == 0 ? "grad_vf1 needs at least 2 rows" : "the second-order flux needs at least 4 rows");
+    return IB_ERR_ARG;
+  }
+  if (inner == 0) return IB_OK;
+  return launch_upwind(f, v, out, n, inner, dx, dt, mode, (hipStream_t)stream);
+}
+
+int ib_fast_sweep(double* f, const double* gain, const double* rho, const double* vx, int nx,
+                  int nvr, long long nvc, double dx, double nu, double* err, void* stream) {
+  if (nx < 1 || nvr < 1 || nvc < 1 || !f || !gain || !rho || !vx || !err) {
+    set_error("invalid fast_sweep arguments");
+    return IB_ERR_ARG;
+  }
+  return launch_fast_sweep(f, gain, rho, vx, nx, nvr, nvc, dx, nu, err, (hipStream_t)stream);
 }
 
 int ib_axpy(double* y, const double* x, double a, long long n, void* stream) {

@@ -9,6 +9,11 @@
 //  * k_axpy, k_df_combine: the in-place updates of src/boltzmann.py:84-164.
 //  * k_moments, k_project: FP64 moments (src/state.py:31-52, any d) and the
 //    new conservative L2 projection of Q.
+//  * k_upwind: the flux-form second-order upwind scheme with the van Leer
+//    limiter (src/riemann/upwind.py:6-50), one output point per thread.
+//  * k_fast_sweep: one Gauss-Seidel sweep of the steady kinetic problem
+//    (src/iter_solver.py:38-62); a thread carries one velocity point through
+//    the space recurrence.
 #pragma clang fp contract(off)
 #include <hip/hip_runtime.h>
 
@@ -222,6 +227,128 @@ int launch_moments(const double* f, double* out, long long batch, int dim, int n
 
 int launch_project(double* q, long long batch, int dim, int n, double L, hipStream_t st) {
   hipLaunchKernelGGL(k_project, dim3((unsigned)batch), dim3(256), 0, st, q, dim, n, L);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Flux-form upwind scheme, src/riemann/upwind.py.  Arrays are (n, inner) with
+// the stencil along axis 0.  Every expression keeps numpy's evaluation order.
+constexpr double UPWIND_EPS = 1e-8;  // upwind.py:3
+
+__device__ __forceinline__ double van_leer(double r) {  // upwind.py:16-17
+  return (r + fabs(r)) / (1.0 + fabs(r));
+}
+// np.maximum(v, 0.) / np.minimum(v, 0.): the first operand wins ties and NaN
+__device__ __forceinline__ double npmax0(double v) { return (v >= 0.0 || isnan(v)) ? v : 0.0; }
+__device__ __forceinline__ double npmin0(double v) { return (v <= 0.0 || isnan(v)) ? v : 0.0; }
+
+// F of F_p_2 (upwind.py:22-25) at row i in 1..n-2, with its speed vp
+__device__ __forceinline__ double flux_p(double fm, double f0, double fp, double vp, double dx,
+                                         double dt) {
+  const double r = (f0 - fm) / ((fp - f0) + UPWIND_EPS);
+  const double phi = van_leer(r);
+  return f0 + ((0.5 * phi) * (1.0 - (vp * dt) / dx)) * (fp - f0);
+}
+// F of F_m_2 (upwind.py:32-35)
+__device__ __forceinline__ double flux_m(double fm, double f0, double fp, double vm, double dx,
+                                         double dt) {
+  const double r = (fp - f0) / ((f0 - fm) + UPWIND_EPS);
+  const double phi = van_leer(r);
+  return f0 + ((0.5 * phi) * (-1.0 - (vm * dt) / dx)) * (f0 - fm);
+}
+
+// mode 0 grad_vf1 (out rows n-2), 1 grad_vf2, 2 F_p_2, 3 F_m_2 (out rows n-4).
+// Modes 0, 1: v is (n, inner); modes 2, 3: v is the speed aligned with f[1:-1], (n-2, inner).
+__global__ void k_upwind(const double* __restrict__ f, const double* __restrict__ v,
+                         double* __restrict__ out, long long n, long long inner, double dx,
+                         double dt, int mode) {
+  const long long rows = mode == 0 ? n - 2 : n - 4;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= rows * inner) return;
+  const long long o = gid / inner, j = gid % inner;
+  auto F = [&](long long i) { return f[i * inner + j]; };
+  auto V = [&](long long i) { return v[i * inner + j]; };
+  double res;
+  if (mode == 0) {  // upwind.py:47-50: diff_x_up(vp f) + diff_x_down(vm f)
+    const double up = npmax0(V(o + 1)) * F(o + 1) - npmax0(V(o)) * F(o);
+    const double dn = npmin0(V(o + 2)) * F(o + 2) - npmin0(V(o + 1)) * F(o + 1);
+    res = up + dn;
+  } else if (mode == 1) {  // upwind.py:40-44
+    const double fp2 = flux_p(F(o + 1), F(o + 2), F(o + 3), npmax0(V(o + 2)), dx, dt) -
+                       flux_p(F(o), F(o + 1), F(o + 2), npmax0(V(o + 1)), dx, dt);
+    const double fm2 = flux_m(F(o + 2), F(o + 3), F(o + 4), npmin0(V(o + 3)), dx, dt) -
+                       flux_m(F(o + 1), F(o + 2), F(o + 3), npmin0(V(o + 2)), dx, dt);
+    res = npmax0(V(o + 2)) * fp2 + npmin0(V(o + 2)) * fm2;
+  } else if (mode == 2) {  // F[2:-2] - F[1:-3], speed index i - 1
+    res = flux_p(F(o + 1), F(o + 2), F(o + 3), V(o + 1), dx, dt) -
+          flux_p(F(o), F(o + 1), F(o + 2), V(o), dx, dt);
+  } else {  // F[3:-1] - F[2:-2]
+    res = flux_m(F(o + 2), F(o + 3), F(o + 4), V(o + 2), dx, dt) -
+          flux_m(F(o + 1), F(o + 2), F(o + 3), V(o + 1), dx, dt);
+  }
+  out[gid] = res;
+}
+
+int launch_upwind(const double* f, const double* v, double* out, long long n, long long inner,
+                  double dx, double dt, int mode, hipStream_t st) {
+  const long long total = (mode == 0 ? n - 2 : n - 4) * inner;
+  if (total <= 0) return IB_OK;
+  hipLaunchKernelGGL(k_upwind, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, f, v, out,
+                     n, inner, dx, dt, mode);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fast sweeping, src/iter_solver.py:38-62.  f, gain: (nx, nvr, nvc); rho: (nx,);
+// vx: (nvr,) the v_x of each velocity row.  Rows with v_x > 0 sweep i = 1..nx-2
+// forward, rows with v_x < 0 backward; rows with v_x = 0 and the end cells keep
+// their values.  err (one double, cleared by k_zero1 first) receives
+// max |f_new - f_old| through an integer max on the bits of the non-negative
+// differences (their order equals the order of the values).
+__global__ void k_zero1(double* p) { *p = 0.0; }
+
+__global__ void k_fast_sweep(double* __restrict__ f, const double* __restrict__ gain,
+                             const double* __restrict__ rho, const double* __restrict__ vx, int nx,
+                             int nvr, long long nvc, double dx, double nu, double* err) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nv = (long long)nvr * nvc;
+  if (gid >= nv) return;
+  const double v = vx[gid / nvc];
+  const double vdx = v / dx;
+  double dmax = 0.0;
+  if (v > 0.0) {
+    double prev = f[gid];
+    for (int i = 1; i < nx - 1; ++i) {
+      const long long at = (long long)i * nv + gid;
+      const double nw = (nu * gain[at] + vdx * prev) / (nu * rho[i] + vdx);
+      const double d = fabs(nw - f[at]);
+      dmax = (d > dmax || isnan(d)) ? d : dmax;
+      f[at] = nw;
+      prev = nw;
+    }
+  } else if (v < 0.0) {
+    double prev = f[(long long)(nx - 1) * nv + gid];
+    for (int i = nx - 2; i >= 1; --i) {
+      const long long at = (long long)i * nv + gid;
+      const double nw = (nu * gain[at] - vdx * prev) / (nu * rho[i] - vdx);
+      const double d = fabs(nw - f[at]);
+      dmax = (d > dmax || isnan(d)) ? d : dmax;
+      f[at] = nw;
+      prev = nw;
+    }
+  }
+  if (dmax != 0.0) atomicMax((unsigned long long*)err, (unsigned long long)__double_as_longlong(dmax));
+}
+
+int launch_fast_sweep(double* f, const double* gain, const double* rho, const double* vx, int nx,
+                      int nvr, long long nvc, double dx, double nu, double* err, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, st, err);
+  IB_HIP(hipGetLastError());
+  const long long nv = (long long)nvr * nvc;
+  hipLaunchKernelGGL(k_fast_sweep, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, st, f, gain,
+                     rho, vx, nx, nvr, nvc, dx, nu, err);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }

@@ -6,7 +6,9 @@ src/riemann/advection_1D.py, src/limiters/tvd.py) and records
 BoltzmannSolver1D.df_advection and tvd.limit outputs for seeded inputs into
 transport_golden.npz (inputs and outputs only).  These pin the masked-array
 ratio, the zero-wave handling and the index arithmetic of
-boltzmann.py:177-205, which no closed form fixes bit for bit.
+boltzmann.py:177-205, which no closed form fixes bit for bit.  It also records
+the flux-form scheme of src/riemann/upwind.py (grad_vf2, grad_vf1, F_p_2,
+F_m_2), whose EPS-regularised van Leer ratio no closed form fixes either.
 """
 import pathlib
 import sys
@@ -60,6 +62,24 @@ def main():
     out["limit_wave"] = wave
     out["limit_s"] = s
     out["limit_out"] = tvd.limit(wave.copy(), s, tvd.minmod, 0.1)
+    # flux-form upwind scheme (separate generator: the arrays above stay as they were)
+    from src.riemann import upwind
+    rng2 = np.random.default_rng(77)
+    uf = rng2.random((14, 3, 4))
+    uf[4:7] = 0.3                  # flat rows: ratios of EPS-sized denominators
+    uf[10, 1] = uf[9, 1]
+    uv = rng2.normal(size=(14, 3, 4))
+    uv[2] = 0.0
+    uv1 = rng2.normal(size=(14, 1, 1))
+    udx, udt = 0.05, 0.01
+    out["up_f"], out["up_v"], out["up_v1"] = uf, uv, uv1
+    out["up_dxdt"] = np.array([udx, udt])
+    out["up_grad_vf2"] = upwind.grad_vf2(uf, uv, udx, udt)
+    out["up_grad_vf2_b"] = upwind.grad_vf2(uf, uv1, udx, udt)
+    out["up_grad_vf1"] = upwind.grad_vf1(uf, uv)
+    vp, vm = np.maximum(uv, 0.0), np.minimum(uv, 0.0)
+    out["up_F_p_2"] = upwind.F_p_2(uf, vp[1:-1], udx, udt)
+    out["up_F_m_2"] = upwind.F_m_2(uf, vm[1:-1], udx, udt)
     np.savez(OUT, **out)
     print("wrote", OUT, "cases", case)
 

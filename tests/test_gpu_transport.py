@@ -58,3 +58,100 @@ def test_tvd_errors(pkg, need_gpu):
     assert rc == pkg._lib.IB_ERR_ARG  # nx + 2 < N, as the reference's broadcast fails
     rc, _ = _tvd(lib, np.zeros((10, 8, 8)), v1, 0.1, 2, 0, 0, 1)
     assert rc == pkg._lib.IB_ERR_BC
+
+
+# ---- flux-form upwind scheme (src/riemann/upwind.py) and the fast sweep (src/iter_solver.py)
+
+def test_upwind_flux_golden_bit_exact(pkg, need_gpu):
+    from proj_int_inelastic_amd.riemann import upwind as U
+    f, v, v1 = G["up_f"], G["up_v"], G["up_v1"]
+    dx, dt = G["up_dxdt"]
+    assert np.array_equal(U.grad_vf2(f, v, dx, dt), G["up_grad_vf2"])
+    assert np.array_equal(U.grad_vf2(f, v1, dx, dt), G["up_grad_vf2_b"])
+    assert np.array_equal(U.grad_vf1(f, v), G["up_grad_vf1"])
+    vp, vm = np.maximum(v, 0.0), np.minimum(v, 0.0)
+    assert np.array_equal(U.F_p_2(f, vp[1:-1], dx, dt), G["up_F_p_2"])
+    assert np.array_equal(U.F_m_2(f, vm[1:-1], dx, dt), G["up_F_m_2"])
+
+
+def test_upwind_flux_oracle_large_and_resident(pkg, need_gpu):
+    import torch
+    from proj_int_inelastic_amd.riemann import upwind as U
+    rng = np.random.default_rng(12)
+    f = rng.random((204, 32, 32))
+    f[50:60] = 0.5
+    v = np.broadcast_to(np.linspace(-8, 8, 32)[None, :, None], (204, 32, 32)).copy()
+    dx, dt = 0.005, 2.5e-4
+    fd, vd = torch.from_numpy(f).cuda(), torch.from_numpy(v).cuda()
+    g2 = U.grad_vf2(fd, vd, dx, dt)
+    assert g2.is_cuda and g2.shape == (200, 32, 32)
+    assert np.array_equal(g2.cpu().numpy(), T.grad_vf2(f, v, dx, dt))
+    assert np.array_equal(U.grad_vf1(fd, vd).cpu().numpy(), T.grad_vf1(f, v))
+    with pytest.raises(NotImplementedError):
+        U.F_p_2(f, v[1:-1], dx, dt, limiter=lambda r: r)
+    with pytest.raises(ValueError):
+        U.grad_vf2(f, v[1:], dx, dt)
+
+
+def test_fast_sweep_kernel_bit_exact(pkg, need_gpu):
+    """ib_fast_sweep against the oracle sweep on identical inputs (gain, rho given)."""
+    import torch
+    from oracle import iter_solver as IS
+    lib = pkg._lib.load()
+    rng = np.random.default_rng(8)
+    nx, N = 64, 32
+    f = rng.random((nx, N, N)) + 0.05
+    gain = rng.random((nx, N, N))
+    rho = rng.random(nx) + 0.5
+    vx = -8 + (np.arange(N) + 0.5) * 0.5
+    vx[3] = 0.0
+    dx, nu = 1.0 / nx, 1 / np.sqrt(np.pi / 2) / 0.1
+    ref = f.copy()
+    err_ref = IS.sweep(ref, gain, rho, vx, dx, nu)
+    fd = torch.from_numpy(f).cuda()
+    gd, rd, vd = (torch.from_numpy(a).cuda() for a in (gain, rho, vx))
+    err = torch.empty(1, dtype=torch.float64, device="cuda")
+    rc = lib.ib_fast_sweep(fd.data_ptr(), gd.data_ptr(), rd.data_ptr(), vd.data_ptr(), nx, N, N, dx,
+                           nu, err.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    assert rc == 0
+    assert np.array_equal(fd.cpu().numpy(), ref)
+    assert err.item() == err_ref
+
+
+def test_fast_sweep_solver_matches_oracle(pkg, need_gpu):
+    """BoltzmannFastSweep.step and solve (n_iter) on the 1D x 2D grid against the oracle
+    sweep driven by the oracle's gain-only Q+ (FP64 <= 1e-12 relative)."""
+    import types
+    from oracle import collision as OC
+    from oracle import iter_solver as IS
+    from oracle import quadrature as OQ
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    from proj_int_inelastic_amd.iter_solver import BoltzmannFastSweep
+    from proj_int_inelastic_amd.solution import Solution
+    N, nr, M, s, nx = 16, 8, 8, 2.0, 12
+    ccfg = {"collision-model": {"dim": 2, "e": 0.9, "e0": 0.9, "gamma": 1.0},
+            "velocity-mesh": {"nv": N, "s": s, "nr": nr}, "circle-quad-rule": {"nphi": M}}
+    sm = SpectralMesh(ccfg)
+    coll = FSInelasticVHSCollision(ccfg, sm)
+    L = sm.L
+    cfg = types.SimpleNamespace(dim_x=1, dim_v=2, xmin=0.0, xmax=1.0, l=L, nx=nx, nv=N)
+    sol = Solution(cfg)
+    v = -L + (np.arange(N) + 0.5) * (2 * L / N)
+    vxg, vyg = np.meshgrid(v, v, indexing="ij")
+    rho = 1.0 + 0.3 * np.sin(np.linspace(0, np.pi, nx))
+    f0 = rho[:, None, None] / (2 * np.pi) * np.exp(-(vxg ** 2 + vyg ** 2) / 2)[None]
+    sol.state.f = f0.copy()
+    solver = BoltzmannFastSweep(None, collision=coll, kn=0.5, tol=1e-30)
+    sig, ws = OQ.circle_rule(M)
+    op = OC.OracleOperator(2, N, s, nr, sig, ws, e=0.9, e0=0.9, gamma=1.0)
+    ref = f0.copy()
+    dv = 2 * L / N
+    for it in range(3):
+        err = solver.step(sol.state)
+        err_ref = IS.fast_sweep_step(ref, lambda g: op.collide(g, with_loss=False), v, 1.0 / nx, dv, 0.5)
+        got = sol.state.f.cpu().numpy()
+        rel = np.abs(got - ref).max() / np.abs(ref).max()
+        assert rel <= 1e-12, (it, rel)
+        assert abs(err - err_ref) <= 1e-12 * max(err_ref, 1e-300), (it, err, err_ref)
+    err, steps = solver.solve(sol, n_iter=3)
+    assert steps == 3 and err > 0
