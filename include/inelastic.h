@@ -140,6 +140,18 @@ int ib_upwind_flux(const double* f, const double* v, double* out, long long n, l
 int ib_fast_sweep(double* f, const double* gain, const double* rho, const double* vx, int nx,
                   int nvr, long long nvc, double dx, double nu, double* err, void* stream);
 
+/* The TVD transport step of ib_tvd1d along the middle axis of (outer, n, inner)
+ * float64 arrays: the speed of column j is v[j % vmod]; ghost cells follow the
+ * BC codes of that axis; accumulate != 0 adds the increment to df instead of
+ * storing it.  Two calls (x with outer = 1, then y with accumulate = 1) give the
+ * dimension-split increment df_x + df_y of a 2D-space grid f (nx, ny, N, N).
+ * New (the reference keeps ghost cells for 2D space, src/solver.py:51-105 and
+ * src/state.py:95-122, but has no 2D df_advection); per axis it is the step of
+ * src/boltzmann.py:177-205. */
+int ib_tvd_axis(const double* f, double* df, long long outer, int n, long long inner,
+                const double* v, long long vmod, double dtdx, int order, int limiter,
+                int bc_lower, int bc_upper, int accumulate, void* stream);
+
 /* y <- y + a x over n float64 values.  Replaces the `state.f += M * df` updates of
  * src/boltzmann.py:84,126-128,137-140,153-158. */
 int ib_axpy(double* y, const double* x, double a, long long n, void* stream);

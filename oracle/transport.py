@@ -10,7 +10,10 @@ Follows the reference:
   * wave propagation``src/boltzmann.py:177-205`` (LeVeque 2nd-order correction)
   * flux-form upwind ``src/riemann/upwind.py:6-50`` (van Leer, unwired there)
 
-New: the MC limiter phi = max(0, min((1+theta)/2, 2, 2 theta)).
+New: the MC limiter phi = max(0, min((1+theta)/2, 2, 2 theta)), and the
+dimension-split 2D-space increment df_x + df_y (df_advection_2d), pinned by
+reduction to the 1D step for data constant along one axis and by exact
+transport of piecewise constant data at CFL 1 along either axis.
 
 Pins (tests/test_oracle_transport_integrators.py, tests/test_golden_transport.py):
 exact transport of piecewise constant data at CFL 1, discrete conservation
@@ -159,3 +162,20 @@ def grad_vf1(f, v):
     vm = np.minimum(v, 0)
     up, dn = vp * f, vm * f
     return (up[1:-1] - up[:-2]) + (dn[2:] - dn[1:-1])
+
+
+def df_advection_2d(f, vx, vy, dt, dx, dy, bcx=(EXTRAP, EXTRAP), bcy=(EXTRAP, EXTRAP), order=2,
+                    limiter=minmod):
+    """New (the reference has no 2D df_advection): dimension-split increment
+    df_x + df_y of f (nx, ny, N, N), each the 1D step of boltzmann.py:177-205
+    along its space axis with speed vx (resp. vy) on the velocity grid.
+    Computed line by line, so the reference's s[LL-1:UL-1] slice acts on the
+    velocity axis as in 1D (harmless for nx + 2 >= N and ny + 2 >= N)."""
+    nx, ny = f.shape[0], f.shape[1]
+    dfx = np.empty_like(f)
+    for j in range(ny):
+        dfx[:, j] = df_advection(f[:, j], vx, dt, dx, bcx[0], bcx[1], order, limiter)
+    dfy = np.empty_like(f)
+    for i in range(nx):
+        dfy[i] = df_advection(f[i], vy, dt, dy, bcy[0], bcy[1], order, limiter)
+    return dfx + dfy

@@ -57,6 +57,7 @@ SIGNATURES = {
     "ib_moments": (_I, [_P, _P, _LL, _I, _I, _D, _P]),
     "ib_project_conservative": (_I, [_P, _LL, _I, _I, _D, _P]),
     "ib_tvd1d": (_I, [_P, _P, _I, _LL, _P, _D, _I, _I, _I, _I, _I, _P]),
+    "ib_tvd_axis": (_I, [_P, _P, _LL, _I, _LL, _P, _LL, _D, _I, _I, _I, _I, _I, _P]),
     "ib_upwind_flux": (_I, [_P, _P, _P, _LL, _LL, _D, _D, _I, _P]),
     "ib_fast_sweep": (_I, [_P, _P, _P, _P, _I, _I, _LL, _D, _D, _P, _P]),
     "ib_axpy": (_I, [_P, _P, _D, _LL, _P]),

@@ -12,6 +12,12 @@ New:
     ``df_advection`` raises there, :166-167); f may carry leading batch axes.
   * ``use_graph = True`` captures one whole outer step (all inner projective
     steps) in a HIP graph after one eager warm-up step and replays it.
+  * ``BoltzmannSolver2D`` for 2D space (f of shape (nx, ny, N, N)): the
+    reference keeps ghost cells for two space dimensions (solver.py:51-105,
+    state.py:95-122) but has no 2D ``df_advection``; here it is the
+    dimension-split sum df_x + df_y of the 1D step along x (speed v_x,
+    ``problem_data["v1"]``) and along y (speed v_y, ``problem_data["v2"]``),
+    each with its own boundary conditions.
 """
 from math import pi
 
@@ -233,4 +239,45 @@ class BoltzmannSolver1D(BoltzmannSolver):
                                         float(dtdx), int(self.order), tvd.limiter_code(self.limiters),
                                         int(lo), int(hi), int(vx_rows), dev.stream_handle()),
                    "ib_tvd1d")
+        return df
+
+
+class BoltzmannSolver2D(BoltzmannSolver):
+    def __init__(self, kn=1, riemann_solver=None, collision_operator=None):
+        self.num_dim_x = 2
+        self._vdev = {}
+        super(BoltzmannSolver2D, self).__init__(kn, riemann_solver, collision_operator)
+
+    def _speed(self, state, key):
+        src = state.problem_data[key]
+        hit = self._vdev.get(key)
+        if hit is None or hit[0] is not src:
+            hit = (src, dev.to_device(src).reshape(-1))
+            self._vdev[key] = hit
+        return hit[1]
+
+    def df_advection(self, state, dt):
+        """df_x + df_y, each the TVD step of boltzmann.py:177-205 along its space axis."""
+        t = dev.torch()
+        rp = getattr(self, "rp", None)
+        if rp is None or getattr(rp, "kernel_rp", getattr(rp, "__name__", None)) != "advection_1D":
+            raise NotImplementedError("df_advection runs the advection_1D Riemann solver on the device")
+        f = state.f
+        if f.dim() < 3:
+            raise ValueError("2D-space f must have shape (nx, ny, velocity...)")
+        nx, ny = f.shape[0], f.shape[1]
+        nvel = f[0, 0].numel()
+        vx, vy = self._speed(state, "v1"), self._speed(state, "v2")
+        if vx.numel() != nvel or vy.numel() != nvel:
+            raise ValueError("v1 and v2 must hold one speed per velocity point")
+        lim = tvd.limiter_code(self.limiters)
+        df = t.empty_like(f)
+        lib = _lib.load()
+        st = dev.stream_handle()
+        for axis, (outer, n, inner, v) in enumerate(((1, nx, ny * nvel, vx), (nx, ny, nvel, vy))):
+            lo, hi = self._bc_codes(axis)
+            dtdx = float(dt / state.grid.delta[axis])
+            _lib.check(lib.ib_tvd_axis(f.data_ptr(), df.data_ptr(), outer, n, inner, v.data_ptr(),
+                                       nvel, dtdx, int(self.order), lim, int(lo), int(hi), axis,
+                                       st), "ib_tvd_axis")
         return df

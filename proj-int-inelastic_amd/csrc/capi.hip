@@ -64,8 +64,9 @@ int launch_tables(Plan* P, const double* d_wr, hipStream_t st);
 int collide_group(Plan** P, int np, const double* const* f, double* const* q,
                   const long long* batch, const int* const* pidx, int with_loss, double hb,
                   hipStream_t st);
-int launch_tvd1d(const double* f, double* df, int nx, long long nv, const double* v1, double dtdx,
-                 int order, int limiter, int bc_lo, int bc_hi, hipStream_t st);
+int launch_tvd1d(const double* f, double* df, long long outer, int nx, long long nv,
+                 const double* v1, long long vmod, double dtdx, int order, int limiter, int bc_lo,
+                 int bc_hi, int accumulate, hipStream_t st);
 int launch_axpy(double* y, const double* x, double a, long long n, hipStream_t st);
 int launch_df_combine(const double* adv, const double* q, double* out, double dt, double kn,
                       long long n, hipStream_t st);
@@ -400,8 +401,33 @@ int ib_tvd1d(const double* f, double* df, int nx, long long nv, const double* v1
     set_error("operands could not be broadcast together (nx + 2 < N)");
     return IB_ERR_ARG;
   }
-  return launch_tvd1d(f, df, nx, nv, v1, dtdx, order, limiter, bc_lower, bc_upper,
+  return launch_tvd1d(f, df, 1, nx, nv, v1, nv, dtdx, order, limiter, bc_lower, bc_upper, 0,
                       (hipStream_t)stream);
+}
+
+int ib_tvd_axis(const double* f, double* df, long long outer, int n, long long inner,
+                const double* v, long long vmod, double dtdx, int order, int limiter,
+                int bc_lower, int bc_upper, int accumulate, void* stream) {
+  if (bc_lower == 0 || bc_upper == 0) {
+    set_error(bc_lower == 0 ? "Lower boundary condition not specified."
+                            : "Upper boundary condition not specified.");
+    return IB_ERR_BC;
+  }
+  if ((bc_lower != 1 && bc_lower != 2) || (bc_upper != 1 && bc_upper != 2)) {
+    set_error("Boundary condition not implemented");
+    return IB_ERR_UNSUPPORTED;
+  }
+  if (outer < 1 || n < 1 || inner < 1 || vmod < 1 || !f || !df || !v ||
+      (order != 1 && order != 2) || (limiter != 0 && limiter != 1)) {
+    set_error("invalid tvd_axis arguments");
+    return IB_ERR_ARG;
+  }
+  if ((bc_lower == 2 || bc_upper == 2) && n < 2) {
+    set_error("periodic boundaries need n >= 2");
+    return IB_ERR_ARG;
+  }
+  return launch_tvd1d(f, df, outer, n, inner, v, vmod, dtdx, order, limiter, bc_lower, bc_upper,
+                      accumulate ? 1 : 0, (hipStream_t)stream);
 }
 
 int ib_upwind_flux(const double* f, const double* v, double* out, long long n, long long inner,

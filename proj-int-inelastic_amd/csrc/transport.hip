@@ -5,7 +5,8 @@
 //    upwind Riemann solver (src/riemann/advection_1D.py:4-21), wave limiting
 //    (src/limiters/tvd.py:6-40) and the wave-propagation update
 //    (src/boltzmann.py:177-205), fused into one stencil: thread (i, v) reads
-//    the five cells i-2..i+2 at velocity v (coalesced over v).
+//    the five cells i-2..i+2 at velocity v (coalesced over v).  The same kernel
+//    runs along either space axis of a 2D-space grid (dimension-split sum).
 //  * k_axpy, k_df_combine: the in-place updates of src/boltzmann.py:84-164.
 //  * k_moments, k_project: FP64 moments (src/state.py:31-52, any d) and the
 //    new conservative L2 projection of Q.
@@ -50,18 +51,24 @@ __device__ __forceinline__ double limited(double wm, double w0, double wp, doubl
   return w0 * zero + (ph * w0) * nz;
 }
 
-__global__ void k_tvd1d(const double* __restrict__ f, double* __restrict__ df, int nx,
-                        long long nv, const double* __restrict__ v1, double dtdx, int order,
-                        int limiter, int bc_lo, int bc_hi) {
+// f, df: (outer, nx, nv) with the stencil along the middle axis; the speed of
+// column v is v1[v % vmod].  accumulate: df += increment (the second sweep of a
+// dimension-split step) instead of df = increment.
+__global__ void k_tvd1d(const double* __restrict__ f, double* __restrict__ df, long long outer,
+                        int nx, long long nv, const double* __restrict__ v1, long long vmod,
+                        double dtdx, int order, int limiter, int bc_lo, int bc_hi,
+                        int accumulate) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long long)nx * nv) return;
-  const int i = (int)(gid / nv);
+  if (gid >= outer * nx * nv) return;
+  const long long o = gid / ((long long)nx * nv);
+  const int i = (int)((gid / nv) % nx);
   const long long v = gid % nv;
   const int J = i + 2;
+  const double* fo = f + o * nx * nv;
   double F[5];
 #pragma unroll
-  for (int t = 0; t < 5; ++t) F[t] = f[ghost_row(J - 2 + t, nx, bc_lo, bc_hi) * nv + v];
-  const double s = v1[v];
+  for (int t = 0; t < 5; ++t) F[t] = fo[ghost_row(J - 2 + t, nx, bc_lo, bc_hi) * nv + v];
+  const double s = v1[v % vmod];
   const double wJm2 = F[1] - F[0];
   const double wJm1 = F[2] - F[1];
   const double wJ = F[3] - F[2];
@@ -80,7 +87,7 @@ __global__ void k_tvd1d(const double* __restrict__ f, double* __restrict__ df, i
     const double F1 = c * l0;
     d = d - dtdx * (F1 - F0);
   }
-  df[gid] = d;
+  df[gid] = accumulate ? df[gid] + d : d;
 }
 
 __global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, double a, long long n) {
@@ -189,11 +196,12 @@ __global__ __launch_bounds__(256) void k_project(double* __restrict__ q, int dim
   }
 }
 
-int launch_tvd1d(const double* f, double* df, int nx, long long nv, const double* v1, double dtdx,
-                 int order, int limiter, int bc_lo, int bc_hi, hipStream_t st) {
-  const long long total = (long long)nx * nv;
-  hipLaunchKernelGGL(k_tvd1d, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, f, df, nx,
-                     nv, v1, dtdx, order, limiter, bc_lo, bc_hi);
+int launch_tvd1d(const double* f, double* df, long long outer, int nx, long long nv,
+                 const double* v1, long long vmod, double dtdx, int order, int limiter, int bc_lo,
+                 int bc_hi, int accumulate, hipStream_t st) {
+  const long long total = outer * nx * nv;
+  hipLaunchKernelGGL(k_tvd1d, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, f, df,
+                     outer, nx, nv, v1, vmod, dtdx, order, limiter, bc_lo, bc_hi, accumulate);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }

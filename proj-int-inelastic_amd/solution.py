@@ -1,5 +1,6 @@
 """Solution (reference ``src/solution.py:27-60``): Domain + State, with
-``problem_data["v1"]`` = v_x on the velocity grid (:57)."""
+``problem_data["v1"]`` = v_x on the velocity grid (:57) and, new, ``"v2"`` = v_y
+(the speed along the second space axis of a 2D-space grid)."""
 from .geometry import Domain
 from .state import State
 
@@ -30,3 +31,8 @@ class Solution(object):
         for _ in range(dim_x):
             v1 = v1[0]
         self.state.problem_data = {"v1": v1}
+        if dim_v >= 2:  # new: v_y, the speed of the second space axis of a 2D-space grid
+            v2 = cc[dim_x + 1]
+            for _ in range(dim_x):
+                v2 = v2[0]
+            self.state.problem_data["v2"] = v2

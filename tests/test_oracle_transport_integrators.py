@@ -91,3 +91,27 @@ def test_evolve_steps_and_divisibility():
         I.num_steps(0.0, 1.0, 0.03)
     f, t = I.evolve(np.array([1.0]), lambda f, dt: -dt * f, "Euler", 0.25, 0.0, 1.0)
     assert t == 1.0 and abs(f[0] - 0.75 ** 4) < 1e-15
+
+
+def test_oracle_2d_split_reduces_to_1d_and_transports_exactly():
+    """df_advection_2d: data constant in y with v_y = 0 gives the 1D increment on every
+    y line; first order at CFL 1 along y (v_x = 0) shifts piecewise constant data by
+    one cell with periodic ghosts."""
+    from oracle import transport as T
+    rng = np.random.default_rng(4)
+    nx, ny, N = 10, 8, 4
+    v = np.linspace(-1.5, 1.5, N)
+    vx = np.broadcast_to(v[:, None], (N, N)).copy()
+    prof = rng.random((nx, N, N))
+    f = np.broadcast_to(prof[:, None], (nx, ny, N, N)).copy()
+    dfy0 = T.df_advection_2d(f, vx, np.zeros((N, N)), 0.01, 0.1, 0.1, (T.PERIODIC,) * 2,
+                             (T.EXTRAP,) * 2)
+    ref = T.df_advection(prof, vx, 0.01, 0.1, T.PERIODIC, T.PERIODIC)
+    for j in range(ny):
+        assert np.array_equal(dfy0[:, j], ref)
+    # exact shift along y at CFL 1 (first order, v_y = +1 everywhere, v_x = 0)
+    g = rng.random((nx, ny, N, N))
+    dy = 0.1
+    df = T.df_advection_2d(g, np.zeros((N, N)), np.ones((N, N)), dy, 0.1, dy, (T.PERIODIC,) * 2,
+                           (T.PERIODIC,) * 2, order=1)
+    np.testing.assert_allclose(g + df, np.roll(g, 1, axis=1), rtol=0, atol=1e-15)
