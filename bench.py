@@ -222,6 +222,59 @@ def uneven_run(precision, nprob):
     return {"solves": len(res), "seconds": dt, "solves_per_s": len(res) / dt}, res
 
 
+def run_2d(nb, steps, warmup, with_fp64):
+    """Side measurement on the 2D 32^2 shape of the homogeneous-cooling and 1D x 2D runs
+    (Nr = 32, M = 8, hard spheres, e = 0.9): nb cells, gain + loss, FP32 fused form
+    (k_fused2) and plain FP64 form; roofline of k_fused2 from its live HIP-event time."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    n2, nr2, m2, s2 = 32, 32, 8, 3.624654714576
+    cfg = {"collision-model": {"dim": 2, "e": 0.9, "e0": 0.9, "gamma": 1.0},
+           "velocity-mesh": {"nv": n2, "s": s2, "nr": nr2}, "circle-quad-rule": {"nphi": m2}}
+    sm = SpectralMesh(cfg)
+    rng = np.random.default_rng(5)
+    dv = 2 * sm.L / n2
+    v = -sm.L + (np.arange(n2) + 0.5) * dv
+    vx, vy = np.meshgrid(v, v, indexing="ij")
+    T = rng.uniform(0.5, 2.0, (nb, 2))
+    f = np.exp(-vx[None] ** 2 / (2 * T[:, 0, None, None]) - vy[None] ** 2 / (2 * T[:, 1, None, None]))
+    f /= f.sum(axis=(1, 2), keepdims=True) * dv * dv
+    f_dev = torch.from_numpy(f).cuda()
+    res = {"workload": "2D 32^2, Nr=32, M=8, hard spheres, e=0.9, gain+loss, %d cells" % nb}
+    flop = flops_per_cell(n2 * n2, nr2, m2)
+    out = {}
+    for prec in (("fp32", "fp64") if with_fp64 else ("fp32",)):
+        c = FSInelasticVHSCollision(cfg, sm, precision=prec)
+        for _ in range(max(warmup, 1)):
+            c.collide(f_dev, with_loss=True)
+        torch.cuda.synchronize()
+        if prec == "fp32":
+            c.set_timing(True)
+        st = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(steps):
+            out[prec] = c.collide(f_dev, with_loss=True)
+        e1.record(st)
+        torch.cuda.synchronize()
+        sec = e0.elapsed_time(e1) / 1e3 / steps
+        res[prec + "_cell_evals_per_s"] = nb / sec
+        if prec == "fp32":
+            kms, kl, ku = c.get_timing()
+            kavg = kms / max(kl, 1)
+            ach = flop * (ku / max(kl, 1)) / (kavg * 1e-3) / 1e12
+            res["roofline_k_fused2"] = {"bound": "valu", "achieved": ach, "peak": 157.3,
+                                        "unit": "TFLOP/s", "frac": ach / 157.3,
+                                        "kernel_avg_ms": kavg, "cells_per_launch": ku / max(kl, 1),
+                                        "flop_per_cell_alg": flop}
+    if "fp64" in out:
+        a, b = out["fp32"], out["fp64"]
+        res["fp32_over_fp64"] = res["fp32_cell_evals_per_s"] / res["fp64_cell_evals_per_s"]
+        res["fp32_vs_fp64_rel_linf_max"] = float(((a - b).abs().amax(dim=(1, 2)) /
+                                                  b.abs().amax(dim=(1, 2))).max())
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,14 +284,17 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--check-cells", type=int, default=2)
     ap.add_argument("--cpu-cells", type=int, default=40)
-    ap.add_argument("--mode", choices=["bench", "profile", "copy"], default="bench",
+    ap.add_argument("--mode", choices=["bench", "profile", "profile2d", "copy"], default="bench",
                     help="profile: FP32 form only on --cells cells (for rocprofv3); "
+                         "profile2d: the 2D 32^2 FP32 form only on --cells-2d cells; "
                          "copy: a plain 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the Poisson-arrival serving run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="arrival rate (1/s)")
+    ap.add_argument("--cells-2d", type=int, default=4096,
+                    help="cells of the 2D 32^2 side measurement (0 = skip)")
     ap.add_argument("--uneven-problems", type=int, default=256,
                     help="problems of the uneven homogeneous batch (0 = skip; 4096 = full size)")
     args = ap.parse_args()
@@ -257,6 +313,9 @@ def main():
         return
     pkg = _load_pkg()
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    if args.mode == "profile2d":
+        print(json.dumps({"mode": "profile2d", **run_2d(args.cells_2d, args.steps, args.warmup, False)}))
+        return
 
     cfg = cfg3()
     sm = SpectralMesh(cfg)
@@ -329,6 +388,8 @@ def main():
         out["fp64_plain_cell_evals_per_s"] = args.cells / res["fp64_s_per_step"]
         out["fp32_over_fp64"] = res["fp64_s_per_step"] / t32
         out["fp32_vs_fp64_gpu_rel_linf_max"] = res["fp32_vs_fp64_gpu_rel_linf_max"]
+    if args.cells_2d > 0:
+        out["shape_2d_32"] = run_2d(args.cells_2d, args.steps, args.warmup, not args.no_fp64)
     if args.uneven_problems > 0:
         u32, r32 = uneven_run("fp32", args.uneven_problems)
         out["uneven_batch"] = {"workload": "homogeneous cooling, 2D 16^2/32^2/64^2 + 3D 16^3/32^3, "
