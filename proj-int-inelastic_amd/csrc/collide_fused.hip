@@ -240,7 +240,10 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     __syncthreads();
     // B: inverse along ky; packed plane of the previous pair
 #pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = W[p][k][l];
+    for (int i = 0; i < N; ++i) {  // in the order the radix-2 stages consume them
+      const int k = bitrev<N>(i);
+      v[k] = W[p][k][l];
+    }
     fft_reg<N, +1>(v);
 #pragma unroll
     for (int k = 0; k < N; ++k) W[p][k][l] = v[k];
@@ -255,7 +258,8 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
         Z[N / 2] = cx<float>{a0.im, b0.im};
       }
 #pragma unroll
-      for (int k = 1; k < N / 2; ++k) {
+      for (int i = 1; i < N / 2; ++i) {
+        const int k = bitrev<N / 2>(i);
         const cx<float> a = W[k][xy][xz], bb = W[k][xy][xz + 1];
         Z[k] = cx<float>{a.re - bb.im, a.im + bb.re};
         Z[N - k] = cx<float>{a.re + bb.im, bb.re - a.im};
@@ -286,7 +290,10 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     __syncthreads();
     // D: forward along ky; X16 of the next pair
 #pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = W[p][k][l];
+    for (int i = 0; i < N; ++i) {  // in the order the radix-2 stages consume them
+      const int k = bitrev<N>(i);
+      v[k] = W[p][k][l];
+    }
     fft_reg<N, -1>(v);
 #pragma unroll
     for (int k = 0; k < N; ++k) W[p][k][l] = v[k];
@@ -294,7 +301,10 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     __syncthreads();
     // E: forward along kz, G * (.) and the sum over angles
 #pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = W[p][l][k];
+    for (int i = 0; i < N; ++i) {
+      const int k = bitrev<N>(i);
+      v[k] = W[p][l][k];
+    }
     fft_reg<N, -1>(v);
     if (p == 0) {
 #pragma unroll
