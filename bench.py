@@ -307,6 +307,7 @@ def main():
                          "copy: a plain 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-kahan", action="store_true", help="skip the fp32k side measurement")
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the Poisson-arrival serving run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="arrival rate (1/s)")
@@ -364,6 +365,14 @@ def main():
         res["fp64_s_per_step"] = t64
         d = (q32 - q64).abs().amax(dim=(1, 2, 3)) / q64.abs().amax(dim=(1, 2, 3))
         res["fp32_vs_fp64_gpu_rel_linf_max"] = float(d.max())
+    if not args.no_kahan:
+        ck = FSInelasticVHSCollision(cfg, sm, precision="fp32k")
+        tk, wk, qk = time_path(ck, f_dev, e, 1, 1 if args.warmup else 0)
+        res["fp32k"] = {"cell_evals_per_s": args.cells / tk}
+        if not args.no_fp64:
+            dk = (qk - q64).abs().amax(dim=(1, 2, 3)) / q64.abs().amax(dim=(1, 2, 3))
+            res["fp32k"]["vs_fp64_gpu_rel_linf_max"] = float(dk.max())
+        del qk
     # error of the FP32 form against the FP64 oracle on a sample of cells
     from oracle import collision as OC
     from oracle import quadrature as OQ
@@ -412,6 +421,8 @@ def main():
                      "flop_per_cell_alg": flop, "kernel_share_of_step": kms / (t32 * 1e3 * args.steps)},
     }
     out["pcie_inclusive_cell_evals_per_s"] = res["pcie_inclusive_cell_evals_per_s"]
+    if "fp32k" in res:
+        out["fp32_kahan"] = dict(res["fp32k"], note="Kahan-compensated angular sums (precision fp32k)")
     if "fp64_s_per_step" in res:
         out["fp64_plain_cell_evals_per_s"] = args.cells / res["fp64_s_per_step"]
         out["fp32_over_fp64"] = res["fp64_s_per_step"] / t32

@@ -26,6 +26,9 @@ extern "C" {
 #define IB_PREC_FP32 1      /* FP32 fast form: FP32 transforms and angular sums,
                                FP64 radial sums, final transform and loss term;
                                fused kernels for 3D 32^3 and 2D 16^2 / 32^2 */
+#define IB_PREC_FP32_KAHAN 2 /* FP32 fast form whose angular sums in the fused
+                               3D 32^3 kernel are Kahan-compensated (other
+                               shapes: as IB_PREC_FP32) */
 
 typedef struct ib_plan ib_plan;
 

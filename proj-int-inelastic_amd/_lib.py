@@ -17,6 +17,7 @@ IB_ERR_UNSUPPORTED = -3
 IB_ERR_BC = -4
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_FP32_KAHAN = 2
 
 
 class ib_plan_desc(ctypes.Structure):

@@ -54,9 +54,9 @@ def key_of(coll, batch):
 
 
 def candidates(coll, batch):
-    if coll.precision == "fp32" and coll.ndim == 3 and coll.sm.nv == 32:
+    if coll.precision in ("fp32", "fp32k") and coll.ndim == 3 and coll.sm.nv == 32:
         return [{"fused_cells": c} for c in (64, 128, 256, 512) if c <= max(64, batch)]
-    if coll.precision == "fp32" and coll.ndim == 2 and coll.sm.nv in (16, 32):
+    if coll.precision in ("fp32", "fp32k") and coll.ndim == 2 and coll.sm.nv in (16, 32):
         return [{"fused_cells": c} for c in (256, 1024, 4096) if c <= max(256, batch)]
     return [{"scratch_budget": b << 20} for b in (256, 1024, 4096)]
 

@@ -9,8 +9,10 @@ the device plan (quadrature, per-restitution kernel tables) instead of the
 reference's (Nr, M, n) complex tables.
 
 Extensions (keyword-only, all defaulting to the reference behaviour):
-  precision   "fp64" (plain FP64 form) or "fp32" (fused FP32 form with
-              Kahan-compensated quadrature sums); constructor keyword.
+  precision   constructor keyword: "fp64" (plain FP64 form), "fp32" (FP32
+              transforms and angular sums, FP64 radial sums, final transform
+              and loss term) or "fp32k" (as "fp32" with Kahan-compensated
+              angular sums in the fused 3D 32^3 kernel).
   with_loss   constructor keyword: make ``__call__`` return gain - loss.
   e, e0       per-cell restitution arrays for ``collide`` (batched problems
               with different e; e0 defaults to e, i.e. constant restitution).
@@ -36,8 +38,8 @@ class FSInelasticVHSCollision(BaseCollision):
 
     def __init__(self, config, spectral_mesh, precision="fp64", with_loss=False, autotune=False,
                  **kwargs):
-        if precision not in ("fp64", "fp32"):
-            raise ValueError("precision must be 'fp64' or 'fp32'")
+        if precision not in ("fp64", "fp32", "fp32k"):
+            raise ValueError("precision must be 'fp64', 'fp32' or 'fp32k'")
         self.precision = precision
         self.autotune = bool(autotune)
         self._tuned = set()
@@ -83,7 +85,8 @@ class FSInelasticVHSCollision(BaseCollision):
         d.m = sigma.shape[0]
         d.L = self.sm.L
         d.gamma = self._gamma
-        d.precision = _lib.PREC_FP64 if self.precision == "fp64" else _lib.PREC_FP32
+        d.precision = {"fp64": _lib.PREC_FP64, "fp32": _lib.PREC_FP32,
+                       "fp32k": _lib.PREC_FP32_KAHAN}[self.precision]
         d.n_param = e_r.shape[0]
         dp = ctypes.POINTER(ctypes.c_double)
         d.r = r.ctypes.data_as(dp)

@@ -163,8 +163,9 @@ int ib_plan_create(const ib_plan_desc* d, ib_plan** out) {
     set_error("nr, m, n_param must be >= 1 and L > 0");
     return IB_ERR_ARG;
   }
-  if (d->precision != IB_PREC_FP64 && d->precision != IB_PREC_FP32) {
-    set_error("precision must be IB_PREC_FP64 or IB_PREC_FP32");
+  if (d->precision != IB_PREC_FP64 && d->precision != IB_PREC_FP32 &&
+      d->precision != IB_PREC_FP32_KAHAN) {
+    set_error("precision must be IB_PREC_FP64, IB_PREC_FP32 or IB_PREC_FP32_KAHAN");
     return IB_ERR_ARG;
   }
   if (!d->r || !d->wr || !d->sigma || !d->wsigma || !d->e_r) {
@@ -178,7 +179,8 @@ int ib_plan_create(const ib_plan_desc* d, ib_plan** out) {
   p->m = d->m;
   p->L = d->L;
   p->gamma = d->gamma;
-  p->precision = d->precision;
+  p->precision = d->precision == IB_PREC_FP64 ? IB_PREC_FP64 : IB_PREC_FP32;
+  p->kahan = d->precision == IB_PREC_FP32_KAHAN;
   p->npts = 1;
   for (int a = 0; a < p->dim; ++a) p->npts *= p->n;
   p->nq = p->dim * (p->n / 2) * (p->n / 2) + 1;

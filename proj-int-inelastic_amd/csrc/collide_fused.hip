@@ -28,12 +28,10 @@
 #include "common.h"
 #include "plan.h"
 
-// 1: Kahan-compensated FP32 sum over the M angles inside a workgroup (needs
-// 64 more registers per thread); 0: plain FP32 sum over the M angles.  The
-// sum over radial nodes is done in FP64 either way (k_f3_reduce).
-#ifndef IB_FUSED_KAHAN
-#define IB_FUSED_KAHAN 0
-#endif
+// The sum over the M angles inside a workgroup is plain FP32 (plan precision
+// IB_PREC_FP32) or Kahan-compensated FP32 (IB_PREC_FP32_KAHAN: 64 more
+// registers per thread, which spill); the sum over radial nodes is FP64 either
+// way (k_f3_reduce).
 
 namespace ib {
 
@@ -52,13 +50,12 @@ constexpr int PAD = N + 1;
 constexpr int THREADS = 512;
 constexpr long long NPTS = (long long)N * N * N;
 constexpr long long PLANE = (long long)N * N;
-constexpr int KP = IB_FUSED_KAHAN ? 2 : 1;  // partial planes per (cell, j): sum (, compensation)
 }  // namespace f3
 
 struct F3Args {
   const cx<float>* fh;  // [cell][17][32][32], kx = 0..16, scaled by 1/n
   const float* f32;     // [cell][32][32][32]
-  cx<float>* part;      // [cell][nr][KP][16][32][32] (sum, and compensation when KP = 2)
+  cx<float>* part;      // [cell][nr][kp][16][32][32]: sum (kp = 1), sum and compensation (kp = 2)
   const int* pidx;
   long long batch;
   long long cell0;
@@ -74,23 +71,23 @@ struct F3Args {
 
 __device__ __forceinline__ cx<float> mulf(cx<float> a, cx<float> b) { return a * b; }
 
-#if IB_FUSED_KAHAN
+// s += x, Kahan-compensated (c carries the lost low-order part) or plain
+template <bool KAHAN>
 __device__ __forceinline__ void acc_add(cx<float>& s, cx<float>& c, cx<float> x) {
-  const float yr = x.re - c.re;
-  const float tr = s.re + yr;
-  c.re = (tr - s.re) - yr;
-  s.re = tr;
-  const float yi = x.im - c.im;
-  const float ti = s.im + yi;
-  c.im = (ti - s.im) - yi;
-  s.im = ti;
+  if constexpr (KAHAN) {
+    const float yr = x.re - c.re;
+    const float tr = s.re + yr;
+    c.re = (tr - s.re) - yr;
+    s.re = tr;
+    const float yi = x.im - c.im;
+    const float ti = s.im + yi;
+    c.im = (ti - s.im) - yi;
+    s.im = ti;
+  } else {
+    (void)c;
+    s = s + x;
+  }
 }
-#else
-__device__ __forceinline__ void acc_add(cx<float>& s, cx<float>& c, cx<float> x) {
-  (void)c;
-  s = s + x;
-}
-#endif
 
 // Hermitian part of a separable factor phi_x(kx) phi_y(ky) phi_z(kz): every
 // factor is Hermitian except its Nyquist entry, so the Hermitian part is
@@ -123,8 +120,10 @@ __device__ __forceinline__ cx<float> sym_at(const SymRow& r, cx<float> fz, bool 
 // Tables are double-buffered so phase C can fill the next pair's while A..E of
 // the current pair read theirs; no thread's work depends on the packed plane's
 // special case beyond a short tail, so no wave runs two code paths in full.
+template <bool KAHAN>
 __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   using namespace f3;
+  constexpr int KP = KAHAN ? 2 : 1;
   __shared__ cx<float> W[NPL][N][PAD];
   __shared__ cx<float> tE[2][3][N];
   __shared__ cx<float> tG[2][3][N];
@@ -151,11 +150,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
 
   cx<float> acc[N];
   cx<float> acc2[2], cmp2[2];
-#if IB_FUSED_KAHAN
-  cx<float> cmp[N];
-#else
-  cx<float> cmp[1];
-#endif
+  cx<float> cmp[KAHAN ? N : 1];
 #pragma unroll
   for (int k = 0; k < N; ++k) acc[k] = cx<float>{0.f, 0.f};
 #pragma unroll
@@ -196,11 +191,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       const cx<float> G0 = (wjp * sp[qq]) * sym_at(r0, tG[b][2][kz], nz);
       const cx<float> G16 = (wjp * sp[qq + (N / 2) * (N / 2)]) * sym_at(r16, tG[b][2][kz], nz);
       const cx<float> t0 = G0 * h0, t1 = G16 * h16;
-#if IB_FUSED_KAHAN
-      acc_add(acc2[h], cmp2[h], cx<float>{t0.re - t1.im, t0.im + t1.re});
-#else
-      acc2[h] = acc2[h] + cx<float>{t0.re - t1.im, t0.im + t1.re};
-#endif
+      acc_add<KAHAN>(acc2[h], cmp2[h], cx<float>{t0.re - t1.im, t0.im + t1.re});
     }
   };
 
@@ -318,7 +309,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       rr.row = ws_half * rr.row;
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        acc_add(acc[k], cmp[IB_FUSED_KAHAN ? k : 0], sym_at(rr, tG[b][2][k], k == N / 2) * v[k]);
+        acc_add<KAHAN>(acc[k], cmp[KAHAN ? k : 0], sym_at(rr, tG[b][2][k], k == N / 2) * v[k]);
         v[k] = fp[k];
       }
     }
@@ -336,19 +327,17 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       const int kzf = kfreq(k, N);
       const float g = cj * sp[q0 + kzf * kzf];
       out[(long long)p * PLANE + (long long)l * N + k] = g * acc[k];
-#if IB_FUSED_KAHAN
-      const cx<float> c = cmp[k];
-      out[NPL * PLANE + (long long)p * PLANE + (long long)l * N + k] = -g * c;
-#endif
+      if constexpr (KAHAN) {
+        const cx<float> c = cmp[k];
+        out[NPL * PLANE + (long long)p * PLANE + (long long)l * N + k] = -g * c;
+      }
     }
   }
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     const int e = tid + h * THREADS;
     out[e] = acc2[h];
-#if IB_FUSED_KAHAN
-    out[NPL * PLANE + e] = cx<float>{-cmp2[h].re, -cmp2[h].im};
-#endif
+    if constexpr (KAHAN) out[NPL * PLANE + e] = cx<float>{-cmp2[h].re, -cmp2[h].im};
   }
 }
 
@@ -372,7 +361,7 @@ __global__ void k_f3_pack(const cx<double>* __restrict__ fhat, const double* __r
 // Sum the per-j partials in FP64 and expand the packed half spectrum to
 // the full FP64 spectrum (Hermitian completion), as the FP64 epilogue expects.
 __global__ void k_f3_reduce(const cx<float>* __restrict__ part, cx<double>* __restrict__ acc64,
-                            long long nb, int nr) {
+                            long long nb, int nr, int kp) {
   using namespace f3;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= nb * NPTS) return;
@@ -380,13 +369,13 @@ __global__ void k_f3_reduce(const cx<float>* __restrict__ part, cx<double>* __re
   const int kx = (int)(i / PLANE), ky = (int)((i / N) % N), kz = (int)(i % N);
   auto sum_at = [&](int pl, int yy, int zz) {
     double re = 0.0, im = 0.0;
-    const cx<float>* b = part + c * nr * KP * NPL * PLANE + (long long)pl * PLANE + (long long)yy * N + zz;
+    const cx<float>* b = part + c * nr * kp * NPL * PLANE + (long long)pl * PLANE + (long long)yy * N + zz;
     for (int j = 0; j < nr; ++j) {
-      const cx<float> hi = b[(long long)j * KP * NPL * PLANE];
+      const cx<float> hi = b[(long long)j * kp * NPL * PLANE];
       re += (double)hi.re;
       im += (double)hi.im;
-      if (KP == 2) {
-        const cx<float> lo = b[(long long)j * KP * NPL * PLANE + NPL * PLANE];
+      if (kp == 2) {
+        const cx<float> lo = b[(long long)j * kp * NPL * PLANE + NPL * PLANE];
         re += (double)lo.re;
         im += (double)lo.im;
       }
@@ -420,7 +409,8 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
   *handled = 1;
   const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 256);
   const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
-  const size_t part_bytes = (size_t)P->nr * KP * NPL * PLANE * sizeof(cx<float>);
+  const int kp = P->kahan ? 2 : 1;
+  const size_t part_bytes = (size_t)P->nr * kp * NPL * PLANE * sizeof(cx<float>);
   const size_t fh32_bytes = (size_t)17 * PLANE * sizeof(cx<float>) + (size_t)NPTS * sizeof(float);
   int rc;
   if ((rc = P->fhat.ensure(sub * b64))) return rc;
@@ -459,11 +449,14 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
     A.pidx = pidx ? pidx + c0 : nullptr;
     const long long groups = (nb + 7) / 8;
     if ((rc = P->timed_begin(st))) return rc;
-    hipLaunchKernelGGL(k_fused3, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
+    if (P->kahan)
+      hipLaunchKernelGGL(k_fused3<true>, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
+    else
+      hipLaunchKernelGGL(k_fused3<false>, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
     hipLaunchKernelGGL(k_f3_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
-                       (cx<double>*)P->acc.ptr, nb, P->nr);
+                       (cx<double>*)P->acc.ptr, nb, P->nr, kp);
     IB_HIP(hipGetLastError());
     if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
                                    with_loss, hb, pidx ? pidx + c0 : nullptr)))

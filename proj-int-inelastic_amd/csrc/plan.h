@@ -32,6 +32,7 @@ struct DevBuf {
 struct Plan {
   // geometry and quadrature (host copies)
   int dim = 0, n = 0, nr = 0, m = 0, precision = 0;
+  bool kahan = false;  // FP32 plan created as IB_PREC_FP32_KAHAN
   double L = 0, gamma = 0;
   long long npts = 0;  // n^dim
   int nq = 0;          // number of |k|^2 values: dim (n/2)^2 + 1

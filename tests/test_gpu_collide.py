@@ -136,10 +136,10 @@ def test_full_3d_32_single_cell_fp64(pkg, need_gpu):
     f = _data(3, 32, sm.L, 1, seed=11)
     refs = {"fp64": op.collide(f, with_loss=True, workers=8),
             "fp32": op.symmetrized().collide(f, with_loss=True, workers=8)}
-    for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
+    for prec, tol in (("fp64", TOL64), ("fp32", TOL32), ("fp32k", TOL32)):
         coll = FSInelasticVHSCollision(cfg, sm, precision=prec)
         q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True).cpu().numpy()
-        err = _err(q, refs[prec])
+        err = _err(q, refs[prec if prec != "fp32k" else "fp32"])
         print("3D 32^3 Nr32 M38", prec, "rel Linf", err)
         assert err <= tol, (prec, err)
 
@@ -218,16 +218,18 @@ def test_batch_invariance_fp64(pkg, need_gpu):
         assert _err(qc, qa[c]) <= 1e-13
 
 
+@pytest.mark.parametrize("prec", ["fp32", "fp32k"])
 @pytest.mark.parametrize("with_loss,hb", [(True, 0.0), (False, 0.1)])
-def test_fused_fp32_3d32_batch(pkg, need_gpu, with_loss, hb):
-    """Fused FP32 form (3D 32^3): 11 cells with per-cell e (a partial XCD group),
-    launched in sub-batches of 4 cells, against the FP64 oracle cell by cell."""
+def test_fused_fp32_3d32_batch(pkg, need_gpu, with_loss, hb, prec):
+    """Fused FP32 form (3D 32^3), plain and Kahan-compensated angular sums: 11 cells
+    with per-cell e (a partial XCD group), launched in sub-batches of 4 cells, against
+    the FP64 oracle cell by cell."""
     import torch
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
     d, N, nr, M, s = 3, 32, 4, 6, 3.624654714576
     cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 1.0)
     sm = SpectralMesh(cfg)
-    coll = FSInelasticVHSCollision(cfg, sm, precision="fp32")
+    coll = FSInelasticVHSCollision(cfg, sm, precision=prec)
     coll.set_tuning(fused_cells=4)
     es = np.linspace(0.5, 1.0, 11)
     f = _data(d, N, sm.L, 11, seed=21)
@@ -237,7 +239,7 @@ def test_fused_fp32_3d32_batch(pkg, need_gpu, with_loss, hb):
         op = _oracle(d, N, nr, M, s, es[c], es[c], 1.0).symmetrized()
         ref = op(f[c], heat_bath=hb) if not with_loss else op.collide(f[c], with_loss=True)
         worst = max(worst, _err(q[c], ref))
-    print("fused FP32 3D32 worst rel Linf", worst)
+    print("fused", prec, "3D32 worst rel Linf", worst)
     assert worst <= TOL32, worst
 
 
