@@ -196,6 +196,67 @@ def serve_run(precision, nreq, rate, seed=5):
     return serving.latency_stats(res)
 
 
+def serve_sweep_requests(precision, nreq, rate, seed):
+    """Requests of the swept-rate serving run, shaped like the serving workload: Poisson
+    arrivals mixing homogeneous cooling solves distributed as the uneven batch (2D 16^2 /
+    32^2 / 64^2, 3D 16^3 / 32^3, TPI2, t_final in {0.5, 0.75, .., 3}) with 1D x 2D jobs
+    (Nx in {64, 128, 256}, 32^2 velocity, eps in {1e-2, 1e-3}, Sod-type Maxwellian data,
+    hard spheres, e ~ U[0.5, 1]).  The 1D jobs use a projective forward Euler schedule
+    that is stable at every Nx (2 inner steps of 6e-5, outer 2.5e-4, inner CFL <= 0.12)
+    and final times 2.5e-4 x U{200..1200}, i.e. 0.05 .. 0.3."""
+    from proj_int_inelastic_amd import serving
+    rng = np.random.default_rng(seed)
+    arrivals = serving.poisson_arrivals(nreq, rate, rng)
+    shapes = [(2, 16, 8), (2, 32, 8), (2, 64, 8), (3, 16, 38), (3, 32, 38)]
+    reqs = []
+    for i, t_arr in enumerate(arrivals):
+        if rng.random() < 0.75:
+            d, n, m = shapes[int(rng.integers(0, len(shapes)))]
+            s = 3.0 if n < 32 else S3
+            L = 0.5 * (3 + math.sqrt(2)) * s
+            dv = 2 * L / n
+            v = -L + (np.arange(n) + 0.5) * dv
+            g = np.meshgrid(*([v] * d), indexing="ij")
+            c = rng.normal(0, 0.1, (2, d))
+            f0 = sum(np.exp(-sum((g[a] - sg * (1.0 if a == 0 else 0.0) - c[k, a]) ** 2
+                                 for a in range(d)) / (2 * 0.5)) for k, sg in enumerate((1, -1)))
+            f0 /= f0.sum() * dv ** d
+            reqs.append(serving.Request(rid=i, f0=f0, t_final=0.25 * int(rng.integers(2, 13)),
+                                        e=float(rng.uniform(0.2, 1.0)), dim=d, n=n, nr=n, m=m, s=s,
+                                        gamma=float(rng.integers(0, 2)), precision=precision,
+                                        integrator="TPI2", inner_steps=(2, 2), inner_dt=(0.01, 0.05),
+                                        dt=0.25, arrival=float(t_arr)))
+        else:
+            nx, n, s = int(rng.choice([64, 128, 256])), 32, S3
+            L = 0.5 * (3 + math.sqrt(2)) * s
+            v = -L + (np.arange(n) + 0.5) * 2 * L / n
+            vx, vy = np.meshgrid(v, v, indexing="ij")
+            x = (np.arange(nx) + 0.5) / nx
+            rho = np.where(x < 0.5, 1.0, 0.125)[:, None, None]
+            T = np.where(x < 0.5, 1.0, 0.8)[:, None, None]
+            f0 = rho / (2 * math.pi * T) * np.exp(-(vx ** 2 + vy ** 2) / (2 * T))
+            reqs.append(serving.Request(rid=i, f0=f0, t_final=2.5e-4 * int(rng.integers(200, 1201)),
+                                        e=float(rng.uniform(0.5, 1.0)), dim=2, n=n, nr=n, m=8, s=s,
+                                        gamma=1.0, kn=float(rng.choice([1e-2, 1e-3])),
+                                        precision=precision, integrator="PFE", inner_steps=(2,),
+                                        inner_dt=(6e-5,), dt=2.5e-4, xlen=1.0, arrival=float(t_arr)))
+    return reqs
+
+
+def serve_sweep(rates, nreq, with_fp64):
+    from proj_int_inelastic_amd import serving
+    out = []
+    for k, rate in enumerate(rates):
+        row = {"rate_per_s": rate, "requests": nreq}
+        for prec in (("fp32", "fp64") if with_fp64 else ("fp32",)):
+            srv = serving.Server(hbm_budget=32 << 30)
+            res = srv.run(serve_sweep_requests(prec, nreq, rate, seed=100 + k), realtime=True)
+            row[prec] = serving.latency_stats(res)
+            print("serve sweep rate %g %s %s" % (rate, prec, row[prec]), file=sys.stderr, flush=True)
+        out.append(row)
+    return out
+
+
 def uneven_requests(nprob, precision, seed=9):
     """The uneven batch of homogeneous cooling problems.
 
@@ -311,6 +372,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="requests of the Poisson-arrival serving run (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="arrival rate (1/s)")
+    ap.add_argument("--serve-sweep", default="",
+                    help="comma-separated arrival rates: run only the swept-rate serving workload")
+    ap.add_argument("--serve-sweep-requests", type=int, default=32)
     ap.add_argument("--cells-2d", type=int, default=4096,
                     help="cells of the 2D 32^2 side measurement (0 = skip)")
     ap.add_argument("--uneven-problems", type=int, default=256,
@@ -331,6 +395,11 @@ def main():
         return
     pkg = _load_pkg()
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    if args.serve_sweep:
+        rates = [float(x) for x in args.serve_sweep.split(",") if x]
+        print(json.dumps({"mode": "serve_sweep", "sweep": serve_sweep(rates, args.serve_sweep_requests,
+                                                                      not args.no_fp64)}))
+        return
     if args.mode == "profile2d":
         print(json.dumps({"mode": "profile2d", **run_2d(args.cells_2d, args.steps, args.warmup, False)}))
         return
