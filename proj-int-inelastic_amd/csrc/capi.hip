@@ -61,6 +61,7 @@ int collide_plain(Plan* P, const double* f, double* q, long long batch, const in
 int collide_fused(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                   int with_loss, double hb, hipStream_t st, int* handled);
 int launch_tables(Plan* P, const double* d_wr, hipStream_t st);
+int launch_phase_tables2(Plan* P, hipStream_t st);
 int collide_group(Plan** P, int np, const double* const* f, double* const* q,
                   const long long* batch, const int* const* pidx, int with_loss, double hb,
                   hipStream_t st);
@@ -91,6 +92,8 @@ static void free_param_tables(Plan* p) {
   if (p->d_crad) (void)hipFree(p->d_crad);
   if (p->d_lam) (void)hipFree(p->d_lam);
   if (p->d_lam64) (void)hipFree(p->d_lam64);
+  if (p->d_gphase) (void)hipFree(p->d_gphase);
+  p->d_gphase = nullptr;
   p->d_a = nullptr;
   p->d_spg = nullptr;
   p->d_crad = nullptr;
@@ -126,6 +129,13 @@ static int set_params(Plan* p, int n_param, const double* e_r, const double* gam
   IB_HIP(hipMalloc((void**)&p->d_lam64, (size_t)n_param * p->nq * sizeof(double)));
   rc = launch_tables(p, d_wr, st);
   if (rc) return rc;
+  if (p->dim == 2 && p->precision == IB_PREC_FP32 && (p->n == 16 || p->n == 32)) {
+    const size_t per = (size_t)p->nr * p->m * 2 * p->n * 2 * sizeof(float);
+    if (!p->d_ephase) IB_HIP(hipMalloc(&p->d_ephase, per));
+    if (n_param <= Plan::kMaxPhaseSets) IB_HIP(hipMalloc(&p->d_gphase, (size_t)n_param * per));
+    rc = launch_phase_tables2(p, st);
+    if (rc) return rc;
+  }
   return IB_OK;
 }
 
@@ -277,6 +287,7 @@ int ib_plan_destroy(ib_plan* p) {
   if (p->d_rj) (void)hipFree(p->d_rj);
   if (p->d_wr) (void)hipFree(p->d_wr);
   if (p->d_wsig) (void)hipFree(p->d_wsig);
+  if (p->d_ephase) (void)hipFree(p->d_ephase);
   p->fhat.release();
   p->acc.release();
   p->work.release();

@@ -26,6 +26,8 @@ int staged_epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void
 
 struct F2Args {
   const cx<float>* fh;  // [cell][N][N], fhat / n
+  const cx<float>* ephase;  // [pair][2][N] or null
+  const cx<float>* gphase;  // [param][pair][2][N] or null
   const float* f32;     // [cell][N][N]
   cx<float>* part;      // [cell][wpc][N][N]
   const int* pidx;
@@ -78,17 +80,31 @@ __global__ __launch_bounds__(F2Shape<N>::THREADS) void k_fused2(F2Args A) {
     const bool act = pair < npairs;
     const int j = act ? pair / A.m : 0, mm = act ? pair % A.m : 0;
     if (act) {
-      const double rj = A.rj[j];
-      const double aj = A.a[ps * A.nr + j] * rj;
-      const double kf = (double)kl;
+      if (A.ephase) {
+        const cx<float>* e = A.ephase + (long long)pair * 2 * N;
+        tab[g][0][l] = e[l];
+        tab[g][1][l] = e[N + l];
+      } else {
+        const double rj = A.rj[j];
 #pragma unroll
-      for (int ax = 0; ax < 2; ++ax) {
-        const double s = A.sigma[mm * 2 + ax];
-        double sn, cs;
-        sincos(-(M_PI * rj * s * kf) / A.L, &sn, &cs);
-        tab[g][ax][l] = cx<float>{(float)cs, (float)sn};
-        sincos(aj * s * kf, &sn, &cs);
-        tab[g][2 + ax][l] = cx<float>{(float)cs, (float)sn};
+        for (int ax = 0; ax < 2; ++ax) {
+          double sn, cs;
+          sincos(-(M_PI * rj * A.sigma[mm * 2 + ax] * (double)kl) / A.L, &sn, &cs);
+          tab[g][ax][l] = cx<float>{(float)cs, (float)sn};
+        }
+      }
+      if (A.gphase) {
+        const cx<float>* gt = A.gphase + ((long long)ps * npairs + pair) * 2 * N;
+        tab[g][2][l] = gt[l];
+        tab[g][3][l] = gt[N + l];
+      } else {
+        const double aj = A.a[ps * A.nr + j] * A.rj[j];
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax) {
+          double sn, cs;
+          sincos(aj * A.sigma[mm * 2 + ax] * (double)kl, &sn, &cs);
+          tab[g][2 + ax][l] = cx<float>{(float)cs, (float)sn};
+        }
       }
     }
     __syncthreads();
@@ -138,6 +154,42 @@ __global__ __launch_bounds__(F2Shape<N>::THREADS) void k_fused2(F2Args A) {
     for (int gg = 0; gg < G; ++gg) s = s + Wg[gg][e / N][e % N];
     out[e] = s;
   }
+}
+
+// Phase tables of every quadrature pair (j, m), axis a and frequency index k:
+// E = exp(-i pi r_j sigma_a kf / L) and, per parameter set, G = exp(i a_j r_j sigma_a kf)
+// (the separable factors of inelastic_collisions.py:97-105), FP64 sincos rounded to FP32.
+__global__ void k_phase_tables2(cx<float>* ephase, cx<float>* gphase, const double* rj,
+                                const double* sigma, const double* a, int nr, int m, int n,
+                                int n_param, double L) {
+  const long long per = (long long)nr * m * 2 * n;
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long total = per * (1 + (gphase ? n_param : 0));
+  if (gid >= total) return;
+  const long long t = gid % per;
+  const int k = (int)(t % n), ax = (int)((t / n) % 2);
+  const long long pair = t / (2LL * n);
+  const int j = (int)(pair / m), mm = (int)(pair % m);
+  const double kf = (double)kfreq(k, n), s = sigma[mm * 2 + ax];
+  double sn, cs;
+  if (gid < per) {
+    sincos(-(M_PI * rj[j] * s * kf) / L, &sn, &cs);
+    ephase[t] = cx<float>{(float)cs, (float)sn};
+  } else {
+    const int ps = (int)(gid / per - 1);
+    sincos(a[ps * nr + j] * rj[j] * s * kf, &sn, &cs);
+    gphase[gid - per] = cx<float>{(float)cs, (float)sn};
+  }
+}
+
+int launch_phase_tables2(Plan* P, hipStream_t st) {
+  const long long per = (long long)P->nr * P->m * 2 * P->n;
+  const long long total = per * (1 + (P->d_gphase ? P->n_param : 0));
+  hipLaunchKernelGGL(k_phase_tables2, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                     (cx<float>*)P->d_ephase, (cx<float>*)P->d_gphase, P->d_rj, P->d_sigma, P->d_a,
+                     P->nr, P->m, P->n, P->n_param, P->L);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
 }
 
 __global__ void k_f2_pack(const cx<double>* __restrict__ fhat, const double* __restrict__ f,
@@ -195,6 +247,8 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
     IB_HIP(hipGetLastError());
     F2Args A{};
     A.fh = fh32;
+    A.ephase = (const cx<float>*)P->d_ephase;
+    A.gphase = (const cx<float>*)P->d_gphase;
     A.f32 = f32;
     A.part = part;
     A.pidx = pc;

@@ -48,6 +48,13 @@ struct Plan {
   void* d_spg = nullptr;      // [n_param][nr][nq]  sp(a_j r_j |k|), T
   void* d_lam = nullptr;      // [n_param][nq]  sum_j |S| r^(g+d-1) w_j sp(pi r_j |k| / L), T
   double* d_lam64 = nullptr;  // [n_param][nq]  the same in FP64 (FP64 epilogue of the fused form)
+  // 1D phase factors per quadrature pair of the 2D fused FP32 form (float2):
+  // E [nr m][2][n] = exp(-i pi r_j sigma_a k / L); G [n_param][nr m][2][n] =
+  // exp(i a_j r_j sigma_a k), built only for n_param <= kMaxPhaseSets (else null:
+  // the kernel evaluates G itself)
+  void* d_ephase = nullptr;
+  void* d_gphase = nullptr;
+  static constexpr int kMaxPhaseSets = 64;
   std::vector<double> gamma_p;  // per parameter set
   // device scratch (grow-only)
   DevBuf fhat, acc, work, aux;

@@ -270,6 +270,25 @@ def test_fused_fp32_2d_batch(pkg, need_gpu, N, nr, M, s, with_loss, hb):
     assert worst <= TOL32, worst
 
 
+def test_fused_fp32_2d_many_parameter_sets(pkg, need_gpu):
+    """More parameter sets than the precomputed phase tables hold (70 > 64): the 2D fused
+    kernel evaluates its gain phase factors itself; checked against the oracle."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    d, N, nr, M, s = 2, 16, 8, 8, 2.0
+    cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 1.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision="fp32")
+    es = np.linspace(0.2, 1.0, 70)
+    f = _data(d, N, sm.L, 70, seed=70)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True, e=es).cpu().numpy()
+    worst = 0.0
+    for c in (0, 17, 40, 69):
+        ref = _oracle(d, N, nr, M, s, es[c], es[c], 1.0).collide(f[c], with_loss=True)
+        worst = max(worst, _err(q[c], ref))
+    assert worst <= TOL32, worst
+
+
 def test_fused_fp32_smooth_data_vs_c2c(pkg, need_gpu):
     """On grid-resolved data the FP32 form meets 1e-5 against the reference's C2C operator."""
     import torch
