@@ -108,25 +108,37 @@ __device__ __forceinline__ cx<float> sym_at(const SymRow& r, cx<float> fz, bool 
   return nz ? (r.nyq * fz).re * r.rest : r.row * fz;
 }
 
-// Pair schedule (4 workgroup barriers per quadrature pair):
+// Orders a wave's LDS writes before its later LDS reads of other lanes' data: a
+// wave's LDS instructions execute in program order, so only the compiler must be
+// kept from moving them.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Pair schedule (2 workgroup barriers per quadrature pair).  Wave w owns the
+// kx planes 2w and 2w + 1 in the kz and ky passes, so A -> B and D -> E exchange
+// data inside a wave only:
 //   A  E * fhat and the inverse kz transform (pencil (p, ky) in registers); the
-//      packed plane p = 0 adds i * X16 staged by phase D of the previous pair
-//   B  inverse ky; all threads: G * (.) and the sum for the packed plane of the
-//      previous pair (2 elements each, read from SP with its (-ky, -kz) mirror)
-//   C  along x (C2R, * f, R2C); threads 0..191: phase tables of the next pair
-//   D  forward ky; all threads: X16 = E16 * fhat16 of the next pair
+//      packed plane p = 0 adds i * X16 staged in phase C of the previous pair
+//   B  inverse ky                                             -- barrier --
+//   C  along x (C2R, * f, R2C); side work of all threads: G * (.) and the sum
+//      for the packed plane of the previous pair (from SP with its (-ky, -kz)
+//      mirror), the phase tables two pairs ahead, X16 of the next pair
+//                                                             -- barrier --
+//   D  forward ky
 //   E  forward kz, then G * (.) and the sum (p >= 1, registers) or the packed
 //      plane parked in SP (p = 0)
-// Tables are double-buffered so phase C can fill the next pair's while A..E of
-// the current pair read theirs; no thread's work depends on the packed plane's
-// special case beyond a short tail, so no wave runs two code paths in full.
+// The phase tables are four-buffered (pair mm uses buffer mm mod 4), so phase C
+// fills one that no phase of the same barrier interval reads.
 template <bool KAHAN>
 __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   using namespace f3;
   constexpr int KP = KAHAN ? 2 : 1;
   __shared__ cx<float> W[NPL][N][PAD];
-  __shared__ cx<float> tE[2][3][N];
-  __shared__ cx<float> tG[2][3][N];
+  __shared__ cx<float> tE[4][3][N];
+  __shared__ cx<float> tG[4][3][N];
   __shared__ float sp[3 * (N / 2) * (N / 2) + 1];
   __shared__ cx<float> S16[N][PAD];
   __shared__ cx<float> SP[N][PAD];
@@ -196,6 +208,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   };
 
   make_tables(0, 0);
+  if (A.m > 1) make_tables(1, 1);
   __syncthreads();
   make_x16(0);
   __syncthreads();
@@ -209,7 +222,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
 #pragma unroll
   for (int k = 0; k < N; ++k) v[k] = fp[k];
   for (int mm = 0; mm < A.m; ++mm) {
-    const int b = mm & 1;
+    const int b = mm & 3;
     const float wj = (float)(crad_j * A.wsig[mm]);
     const float ws_half = (float)(0.5 * A.wsig[mm]);
     // A: E * fhat, inverse along kz
@@ -228,8 +241,8 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
 #pragma unroll
       for (int k = 0; k < N; ++k) W[p][l][k] = v[k];
     }
-    __syncthreads();
-    // B: inverse along ky; packed plane of the previous pair
+    wave_sync();  // planes p = 2w, 2w + 1 belong to wave w: phase B reads only them
+    // B: inverse along ky
 #pragma unroll
     for (int i = 0; i < N; ++i) {  // in the order the radix-2 stages consume them
       const int k = bitrev<N>(i);
@@ -238,7 +251,6 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     fft_reg<N, +1>(v);
 #pragma unroll
     for (int k = 0; k < N; ++k) W[p][k][l] = v[k];
-    if (mm > 0) packed_acc(b ^ 1, wj_prev);
     __syncthreads();
     // C: along x, two real pencils (xy, xz), (xy, xz + 1) as one complex FFT
     {
@@ -277,9 +289,14 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
         W[k][xy][xz + 1] = cx<float>{zk.im + zm.im, zm.re - zk.re};
       }
     }
-    if (mm + 1 < A.m) make_tables(mm + 1, b ^ 1);
+    // side work: the packed plane of the previous pair, the phase tables two pairs
+    // ahead and X16 of the next pair (tables are four-buffered, so none of these
+    // touches a buffer another phase of this barrier interval reads)
+    if (mm > 0) packed_acc((mm - 1) & 3, wj_prev);
+    if (mm + 2 < A.m) make_tables(mm + 2, (mm + 2) & 3);
+    if (mm + 1 < A.m) make_x16((mm + 1) & 3);
     __syncthreads();
-    // D: forward along ky; X16 of the next pair
+    // D: forward along ky
 #pragma unroll
     for (int i = 0; i < N; ++i) {  // in the order the radix-2 stages consume them
       const int k = bitrev<N>(i);
@@ -288,8 +305,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     fft_reg<N, -1>(v);
 #pragma unroll
     for (int k = 0; k < N; ++k) W[p][k][l] = v[k];
-    if (mm + 1 < A.m) make_x16(b ^ 1);
-    __syncthreads();
+    wave_sync();
     // E: forward along kz, G * (.) and the sum over angles
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -316,7 +332,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     wj_prev = wj;
   }
   __syncthreads();
-  packed_acc((A.m - 1) & 1, wj_prev);
+  packed_acc((A.m - 1) & 3, wj_prev);
 
   cx<float>* out = A.part + ((A.cell0 + cell) * A.nr + j) * KP * NPL * PLANE;
   if (p != 0) {
