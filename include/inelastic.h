@@ -106,11 +106,12 @@ int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, 
 int ib_laplacian(ib_plan* plan, const double* f, double* out, long long batch, void* stream);
 
 /* FP64 moments per cell: out (batch, dim + 2) = [rho, rho u_1..u_d (momentum), energy],
- * each times dv^dim, v_i = -L + (i + 1/2) dv.  Replaces src/state.py:31-52 (2D only there). */
+ * each times dv^dim, v_i = -L + (i + 1/2) dv, n <= 256.  Replaces src/state.py:31-52 (2D only
+ * there). */
 int ib_moments(const double* f, double* out, long long batch, int dim, int n, double L,
                void* stream);
 
-/* In-place FP64 L2 projection of q (batch, n^dim) onto {sum q = 0, sum q v_a = 0}.
+/* In-place FP64 L2 projection of q (batch, n^dim) onto {sum q = 0, sum q v_a = 0}, n <= 256.
  * New (the reference has no conservative correction). */
 int ib_project_conservative(double* q, long long batch, int dim, int n, double L, void* stream);
 

@@ -372,7 +372,7 @@ int ib_laplacian(ib_plan* p, const double* f, double* out, long long batch, void
 
 int ib_moments(const double* f, double* out, long long batch, int dim, int n, double L,
                void* stream) {
-  if (batch < 0 || (dim != 2 && dim != 3) || n < 1 || (batch > 0 && (!f || !out))) {
+  if (batch < 0 || (dim != 2 && dim != 3) || n < 1 || n > 256 || (batch > 0 && (!f || !out))) {
     set_error("invalid moments arguments");
     return IB_ERR_ARG;
   }
@@ -381,7 +381,7 @@ int ib_moments(const double* f, double* out, long long batch, int dim, int n, do
 }
 
 int ib_project_conservative(double* q, long long batch, int dim, int n, double L, void* stream) {
-  if (batch < 0 || (dim != 2 && dim != 3) || n < 1 || (batch > 0 && !q)) {
+  if (batch < 0 || (dim != 2 && dim != 3) || n < 1 || n > 256 || (batch > 0 && !q)) {
     set_error("invalid projection arguments");
     return IB_ERR_ARG;
   }

@@ -374,16 +374,18 @@ __global__ void k_f3_pack(const cx<double>* __restrict__ fhat, const double* __r
   }
 }
 
-// Sum the per-j partials in FP64 and expand the packed half spectrum to
-// the full FP64 spectrum (Hermitian completion), as the FP64 epilogue expects.
+// Sum the per-j partials in FP64 and expand the packed half spectrum to the
+// full FP64 spectrum (Hermitian completion), as the FP64 epilogue expects.  A
+// thread owns one point of the stored planes (kx = 1..15, or the packed plane)
+// and writes it together with its conjugate mirror, so every partial is read once.
 __global__ void k_f3_reduce(const cx<float>* __restrict__ part, cx<double>* __restrict__ acc64,
                             long long nb, int nr, int kp) {
   using namespace f3;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= nb * NPTS) return;
-  const long long c = gid / NPTS, i = gid % NPTS;
-  const int kx = (int)(i / PLANE), ky = (int)((i / N) % N), kz = (int)(i % N);
-  auto sum_at = [&](int pl, int yy, int zz) {
+  if (gid >= nb * NPL * PLANE) return;
+  const long long c = gid / (NPL * PLANE), i = gid % (NPL * PLANE);
+  const int pl = (int)(i / PLANE), ky = (int)((i / N) % N), kz = (int)(i % N);
+  auto sum_at = [&](int yy, int zz) {
     double re = 0.0, im = 0.0;
     const cx<float>* b = part + c * nr * kp * NPL * PLANE + (long long)pl * PLANE + (long long)yy * N + zz;
     for (int j = 0; j < nr; ++j) {
@@ -399,20 +401,16 @@ __global__ void k_f3_reduce(const cx<float>* __restrict__ part, cx<double>* __re
     return cx<double>{re, im};
   };
   const int my = (N - ky) % N, mz = (N - kz) % N;
-  cx<double> out;
-  if (kx == 0 || kx == N / 2) {
-    const cx<double> P = sum_at(0, ky, kz), Pm = sum_at(0, my, mz);
-    if (kx == 0)
-      out = cx<double>{0.5 * (P.re + Pm.re), 0.5 * (P.im - Pm.im)};
-    else
-      out = cx<double>{0.5 * (P.im + Pm.im), 0.5 * (Pm.re - P.re)};
-  } else if (kx < N / 2) {
-    out = sum_at(kx, ky, kz);
+  cx<double>* out = acc64 + c * NPTS;
+  const cx<double> P = sum_at(ky, kz);
+  if (pl == 0) {  // packed kx = 0 (re) and kx = 16 (im) planes
+    const cx<double> Pm = sum_at(my, mz);
+    out[(long long)ky * N + kz] = cx<double>{0.5 * (P.re + Pm.re), 0.5 * (P.im - Pm.im)};
+    out[(N / 2) * PLANE + (long long)ky * N + kz] = cx<double>{0.5 * (P.im + Pm.im), 0.5 * (Pm.re - P.re)};
   } else {
-    const cx<double> t = sum_at(N - kx, my, mz);
-    out = cx<double>{t.re, -t.im};
+    out[(long long)pl * PLANE + (long long)ky * N + kz] = P;
+    out[(long long)(N - pl) * PLANE + (long long)my * N + mz] = cx<double>{P.re, -P.im};
   }
-  acc64[gid] = out;
 }
 
 int collide_fused(Plan* P, const double* f, double* q, long long batch, const int* pidx,
@@ -471,7 +469,8 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
       hipLaunchKernelGGL(k_fused3<false>, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
-    hipLaunchKernelGGL(k_f3_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
+    const long long half = nb * NPL * PLANE;
+    hipLaunchKernelGGL(k_f3_reduce, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, st, part,
                        (cx<double>*)P->acc.ptr, nb, P->nr, kp);
     IB_HIP(hipGetLastError());
     if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,

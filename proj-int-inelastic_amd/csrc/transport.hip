@@ -120,80 +120,102 @@ __device__ double block_sum(double x, double* sh) {
   return r;
 }
 
-__device__ __forceinline__ double vcoord(long long idx, int axis, int dim, int n, double L) {
-  long long div = 1;
-  for (int a = dim - 1; a > axis; --a) div *= n;
-  const int i = (int)((idx / div) % n);
-  const double dv = 2 * L / n;
-  return -L + (i + 0.5) * dv;
+// Grid coordinates of a cell's flattened velocity index: v_i = -L + (i + 1/2) dv
+// per axis, from a per-workgroup table; lg = log2 n for power-of-two n (shifts),
+// -1 otherwise (32-bit division).
+struct VGrid {
+  int dim, n, lg;
+  int npts;
+  __device__ int index(int i, int a) const {
+    const int rest = dim - 1 - a;
+    if (lg >= 0) return (i >> (lg * rest)) & (n - 1);
+    int div = 1;
+    for (int t = 0; t < rest; ++t) div *= n;
+    return (i / div) % n;
+  }
+};
+
+constexpr int MOM_THREADS = 512;
+
+__device__ void fill_vtab(double* vtab, int n, double L) {
+  for (int i = threadIdx.x; i < n; i += MOM_THREADS) vtab[i] = -L + (i + 0.5) * (2 * L / n);
+  __syncthreads();
 }
 
-// out[c] = [rho, m_1..m_d, E] * dv^d, one 256-thread workgroup per cell.
-__global__ __launch_bounds__(256) void k_moments(const double* __restrict__ f, double* __restrict__ out,
-                                                 int dim, int n, double L) {
-  __shared__ double sh[256];
+// out[c] = [rho, m_1..m_d, E] * dv^d, one workgroup per cell.
+__global__ __launch_bounds__(MOM_THREADS) void k_moments(const double* __restrict__ f,
+                                                         double* __restrict__ out, VGrid g,
+                                                         double L) {
+  __shared__ double sh[MOM_THREADS];
+  __shared__ double vtab[256];
+  fill_vtab(vtab, g.n, L);
   const long long c = blockIdx.x;
-  long long npts = 1;
-  for (int a = 0; a < dim; ++a) npts *= n;
-  const double* fc = f + c * npts;
+  const double* fc = f + c * g.npts;
   double s0 = 0, s1[3] = {0, 0, 0}, s2 = 0;
-  for (long long i = threadIdx.x; i < npts; i += 256) {
+  for (int i = threadIdx.x; i < g.npts; i += MOM_THREADS) {
     const double x = fc[i];
     s0 += x;
     double v2 = 0;
-    for (int a = 0; a < dim; ++a) {
-      const double va = vcoord(i, a, dim, n, L);
+    for (int a = 0; a < g.dim; ++a) {
+      const double va = vtab[g.index(i, a)];
       s1[a] += x * va;
       v2 += va * va;
     }
     s2 += x * v2;
   }
-  const double dv = 2 * L / n;
+  const double dv = 2 * L / g.n;
   double w = 1;
-  for (int a = 0; a < dim; ++a) w *= dv;
-  const double r0 = block_sum<256>(s0, sh);
+  for (int a = 0; a < g.dim; ++a) w *= dv;
+  const double r0 = block_sum<MOM_THREADS>(s0, sh);
   double r1[3];
-  for (int a = 0; a < dim; ++a) r1[a] = block_sum<256>(s1[a], sh);
-  const double r2 = block_sum<256>(s2, sh);
+  for (int a = 0; a < g.dim; ++a) r1[a] = block_sum<MOM_THREADS>(s1[a], sh);
+  const double r2 = block_sum<MOM_THREADS>(s2, sh);
   if (threadIdx.x == 0) {
-    double* o = out + c * (dim + 2);
+    double* o = out + c * (g.dim + 2);
     o[0] = r0 * w;
-    for (int a = 0; a < dim; ++a) o[1 + a] = r1[a] * w;
-    o[dim + 1] = 0.5 * r2 * w;
+    for (int a = 0; a < g.dim; ++a) o[1 + a] = r1[a] * w;
+    o[g.dim + 1] = 0.5 * r2 * w;
   }
 }
 
 // Q <- Q - mean(Q) - sum_a (<Q, v_a> / <v_a, v_a>) v_a, one workgroup per cell.
 // The constraint rows {1, v_a} are mutually orthogonal on the symmetric grid,
 // so the projection needs only these d + 1 reductions.
-__global__ __launch_bounds__(256) void k_project(double* __restrict__ q, int dim, int n, double L) {
-  __shared__ double sh[256];
+__global__ __launch_bounds__(MOM_THREADS) void k_project(double* __restrict__ q, VGrid g,
+                                                         double L) {
+  __shared__ double sh[MOM_THREADS];
+  __shared__ double vtab[256];
+  fill_vtab(vtab, g.n, L);
   const long long c = blockIdx.x;
-  long long npts = 1;
-  for (int a = 0; a < dim; ++a) npts *= n;
-  double* qc = q + c * npts;
+  double* qc = q + c * g.npts;
   double s0 = 0, s1[3] = {0, 0, 0};
-  for (long long i = threadIdx.x; i < npts; i += 256) {
+  for (int i = threadIdx.x; i < g.npts; i += MOM_THREADS) {
     const double x = qc[i];
     s0 += x;
-    for (int a = 0; a < dim; ++a) s1[a] += x * vcoord(i, a, dim, n, L);
+    for (int a = 0; a < g.dim; ++a) s1[a] += x * vtab[g.index(i, a)];
   }
-  const double r0 = block_sum<256>(s0, sh);
+  const double r0 = block_sum<MOM_THREADS>(s0, sh);
   double coef[3] = {0, 0, 0};
   double vv = 0;  // sum over the grid of v_a^2 = n^(d-1) sum_i v_i^2
-  const double dv = 2 * L / n;
-  for (int i = 0; i < n; ++i) {
-    const double vi = -L + (i + 0.5) * dv;
-    vv += vi * vi;
-  }
-  vv *= (double)(npts / n);
-  for (int a = 0; a < dim; ++a) coef[a] = block_sum<256>(s1[a], sh) / vv;
-  const double mean = r0 / (double)npts;
-  for (long long i = threadIdx.x; i < npts; i += 256) {
+  for (int i = 0; i < g.n; ++i) vv += vtab[i] * vtab[i];
+  vv *= (double)(g.npts / g.n);
+  for (int a = 0; a < g.dim; ++a) coef[a] = block_sum<MOM_THREADS>(s1[a], sh) / vv;
+  const double mean = r0 / (double)g.npts;
+  for (int i = threadIdx.x; i < g.npts; i += MOM_THREADS) {
     double x = qc[i] - mean;
-    for (int a = 0; a < dim; ++a) x = x - coef[a] * vcoord(i, a, dim, n, L);
+    for (int a = 0; a < g.dim; ++a) x = x - coef[a] * vtab[g.index(i, a)];
     qc[i] = x;
   }
+}
+
+static VGrid vgrid(int dim, int n) {
+  VGrid g{dim, n, -1, 1};
+  for (int a = 0; a < dim; ++a) g.npts *= n;
+  if ((n & (n - 1)) == 0) {
+    g.lg = 0;
+    while ((1 << g.lg) < n) ++g.lg;
+  }
+  return g;
 }
 
 int launch_tvd1d(const double* f, double* df, long long outer, int nx, long long nv,
@@ -228,13 +250,15 @@ int launch_df_combine(const double* adv, const double* q, double* out, double dt
 
 int launch_moments(const double* f, double* out, long long batch, int dim, int n, double L,
                    hipStream_t st) {
-  hipLaunchKernelGGL(k_moments, dim3((unsigned)batch), dim3(256), 0, st, f, out, dim, n, L);
+  hipLaunchKernelGGL(k_moments, dim3((unsigned)batch), dim3(MOM_THREADS), 0, st, f, out,
+                     vgrid(dim, n), L);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }
 
 int launch_project(double* q, long long batch, int dim, int n, double L, hipStream_t st) {
-  hipLaunchKernelGGL(k_project, dim3((unsigned)batch), dim3(256), 0, st, q, dim, n, L);
+  hipLaunchKernelGGL(k_project, dim3((unsigned)batch), dim3(MOM_THREADS), 0, st, q,
+                     vgrid(dim, n), L);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }
