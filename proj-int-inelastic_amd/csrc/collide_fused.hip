@@ -169,9 +169,13 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   for (int k = 0; k < (int)(sizeof(cmp) / sizeof(cmp[0])); ++k) cmp[k] = cx<float>{0.f, 0.f};
   acc2[0] = acc2[1] = cmp2[0] = cmp2[1] = cx<float>{0.f, 0.f};
 
+  // The side tasks recompute their indices from an opaque copy of tid each pair,
+  // so that none of their addresses occupies a register across the pair loop.
   auto make_tables = [&](int mm, int b) {
-    if (tid < 6 * N) {
-      const int which = tid / (3 * N), ax = (tid % (3 * N)) / N, k = tid % N;
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
+    if (tq < 6 * N) {
+      const int which = tq / (3 * N), ax = (tq % (3 * N)) / N, k = tq % N;
       const double s = A.sigma[mm * 3 + ax];
       const double kf = (double)kfreq(k, N);
       const double th = which == 0 ? -(rpl * s * kf) : (aj * s * kf);
@@ -181,17 +185,21 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     }
   };
   auto make_x16 = [&](int b) {
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int e = tid + h * THREADS, ky = e / N, kz = e % N;
+      const int e = tq + h * THREADS, ky = e / N, kz = e % N;
       const SymRow r16 = sym_row(tE[b][0][N / 2], true, tE[b][1][ky], ky == N / 2);
       S16[ky][kz] = sym_at(r16, tE[b][2][kz], kz == N / 2) * fh[16 * PLANE + e];
     }
   };
   auto packed_acc = [&](int b, float wjp) {
+    int tq = tid;
+    asm volatile("" : "+v"(tq));
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int e = tid + h * THREADS, ky = e / N, kz = e % N;
+      const int e = tq + h * THREADS, ky = e / N, kz = e % N;
       const cx<float> P = SP[ky][kz], Pm = SP[(N - ky) % N][(N - kz) % N];
       const cx<float> h0{0.5f * (P.re + Pm.re), 0.5f * (P.im - Pm.im)};
       const cx<float> h16{0.5f * (P.im + Pm.im), 0.5f * (Pm.re - P.re)};
