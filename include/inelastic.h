@@ -6,6 +6,12 @@
  * returns IB_OK (0) or a negative status; ib_last_error() gives the message
  * of the last failure on the calling thread.
  *
+ * A plan owns grow-only device scratch shared by its calls: calls on one plan
+ * must not overlap (use one stream per plan, or synchronize between streams);
+ * different plans are independent.  A call that needs more scratch than the
+ * plan holds synchronizes the device before it grows the buffer (so a plan's
+ * first call at a new maximum batch size does not belong in a graph capture).
+ *
  * Citations are to the reference interface each entry point replaces
  * (paths relative to the reference repository).
  */

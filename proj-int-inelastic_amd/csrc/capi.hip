@@ -16,7 +16,11 @@ void set_error(const std::string& msg) { g_err = msg; }
 
 int DevBuf::ensure(size_t want) {
   if (want <= bytes) return IB_OK;
-  if (ptr) (void)hipFree(ptr);
+  // growing: work queued earlier on any stream may still read the old buffer
+  if (ptr) {
+    IB_HIP(hipDeviceSynchronize());
+    (void)hipFree(ptr);
+  }
   ptr = nullptr;
   bytes = 0;
   IB_HIP(hipMalloc(&ptr, want));
