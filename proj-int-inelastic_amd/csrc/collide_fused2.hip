@@ -223,8 +223,10 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
   constexpr long long NPTS = (long long)N * N;
   const int npairs = P->nr * P->m;
   const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 4096);
-  int wpc = (int)std::max<long long>(1, (2048 + sub - 1) / sub);
-  wpc = std::min(wpc, std::max(1, (npairs + G - 1) / G));
+  // workgroups per cell fixed by the plan (16 rounds of G pairs each), not by the
+  // batch size: a cell's FP32 summation order, and so its result, is the same in
+  // any batch
+  const int wpc = std::max(1, (npairs + 16 * G - 1) / (16 * G));
   const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
   int rc;
   if ((rc = P->fhat.ensure(sub * b64))) return rc;

@@ -289,6 +289,29 @@ def test_fused_fp32_2d_many_parameter_sets(pkg, need_gpu):
     assert worst <= TOL32, worst
 
 
+@pytest.mark.parametrize("d,N,nr,M,s", [(3, 32, 8, 14, 3.624654714576), (2, 32, 16, 8, 3.0)])
+def test_fused_fp32_batch_invariance_full_size(pkg, need_gpu, d, N, nr, M, s):
+    """Size-independent property at batch sizes beyond the oracle's reach: 300 cells made of
+    10 distinct cells repeated (several sub-batches, every XCD group position, per-cell e) give
+    bit-identical results for every copy, equal to each distinct cell evaluated alone."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 1.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision="fp32")
+    base = _smooth(d, N, sm.L, 10, seed=123)
+    ebase = np.linspace(0.5, 1.0, 10)
+    idx = np.arange(300) % 10
+    f = torch.from_numpy(base[idx]).cuda()
+    q = coll.collide(f, with_loss=True, e=ebase[idx]).cpu().numpy()
+    for c in range(10):
+        copies = q[idx == c]
+        assert np.array_equal(copies, np.broadcast_to(copies[0], copies.shape)), c
+        alone = coll.collide(torch.from_numpy(base[c:c + 1]).cuda(), with_loss=True,
+                             e=ebase[c:c + 1]).cpu().numpy()
+        assert np.array_equal(alone[0], copies[0]), c
+
+
 def test_fused_fp32_smooth_data_vs_c2c(pkg, need_gpu):
     """On grid-resolved data the FP32 form meets 1e-5 against the reference's C2C operator."""
     import torch
