@@ -104,10 +104,12 @@ class Result:
 class _Class:
     """Problems sharing one collision plan and one integrator."""
 
-    def __init__(self, key, proto):
+    def __init__(self, key, proto, autotune=False):
         t = dev.require_gpu()
         self.key = key
         self.proto = proto
+        self.autotune = autotune
+        self._tuned = set()
         d = proto.dim
         cfg = {"collision-model": {"dim": d, "e": 1.0, "e0": 1.0, "gamma": proto.gamma},
                "velocity-mesh": {"nv": proto.n, "s": proto.s, "nr": proto.nr}}
@@ -144,6 +146,17 @@ class _Class:
         self.cell_e = cell_e
         self.cell_gamma = np.concatenate([np.full(a.cells, a.req.gamma) for a in act]) if act else np.zeros(0)
         self.cell_kn = dev.to_device(np.concatenate([np.full(a.cells, a.req.kn) for a in act])) if act else None
+        if self.autotune and act:
+            self._tune()
+
+    def _tune(self):
+        """Pick the launch shape for this class's cell count (once per size bucket;
+        the autotuner's cache makes later servers of this process reuse it)."""
+        from . import autotune as at
+        b = at.bucket(self.f.shape[0])
+        if b not in self._tuned:
+            self._tuned.add(b)
+            at.tune(self.coll, self.f, with_loss=True)
 
     def df_from_q(self, f, q, h):
         """Stacked df of every problem: TVD transport (1D jobs) + h Q / kn."""
@@ -246,8 +259,9 @@ class Server:
     """Admission by HBM budget, continuous batching, latency bookkeeping."""
 
     def __init__(self, hbm_budget=64 << 30, max_cells_per_class=None, clock=time.perf_counter,
-                 use_graph=True):
+                 use_graph=True, autotune=False):
         self.use_graph = use_graph
+        self.autotune = autotune
         self.groups = {}
         self.budget = int(hbm_budget)
         self.used = 0
@@ -300,7 +314,7 @@ class Server:
             new = self._admit(now)
             for key, items in new.items():
                 if key not in self.classes:
-                    self.classes[key] = _Class(key, items[0][0].req)
+                    self.classes[key] = _Class(key, items[0][0].req, self.autotune)
                 cls = self.classes[key]
                 cls.rebuild(cls.active, items)
             busy = [c for c in self.classes.values() if c.active]

@@ -80,3 +80,20 @@ def test_graph_replay_and_mixed_shapes(pkg, need_gpu):
     b = serving.Server(use_graph=False).run(list(reqs), realtime=False)
     for k in a:
         assert np.array_equal(a[k].f, b[k].f), k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_autotuned_server_matches_default(pkg, need_gpu, tmp_path, monkeypatch, precision):
+    """A server that tunes each class's launch shape gives the default server's results."""
+    import dataclasses
+    from proj_int_inelastic_amd import autotune, serving
+    monkeypatch.setenv("INELASTIC_TUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(autotune, "_MEM", {})
+    L = 0.5 * (3 + math.sqrt(2)) * 3.0
+    reqs = [dataclasses.replace(r, precision=precision) for r in _requests(serving.Request, L)]
+    a = serving.Server(autotune=True).run(list(reqs), realtime=False)
+    b = serving.Server().run(list(reqs), realtime=False)
+    assert len(autotune._MEM) >= 2
+    for k in b:
+        err = np.abs(a[k].f - b[k].f).max() / np.abs(b[k].f).max()
+        assert err <= 1e-12, (k, err)
