@@ -257,6 +257,81 @@ def serve_sweep(rates, nreq, with_fp64):
     return out
 
 
+def valu_peak():
+    """Sustained vector-ALU FMA rate (TFLOP/s) in FP32 and FP64, from the probe library's
+    FMA-loop kernel built with the collision library's flags (no packed FP32)."""
+    import ctypes
+    lib = ctypes.CDLL(str(ROOT / "proj-int-inelastic_amd" / "libinelastic_probe.so"))
+    fn = lib.ibp_fma_peak
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    out = {}
+    for dt, name in ((0, "fp32"), (1, "fp64")):
+        fl, ms = ctypes.c_double(), ctypes.c_double()
+        rc = fn(dt, 256 * 64, 512, 5, ctypes.byref(fl), ctypes.byref(ms))
+        if rc != 0:
+            raise RuntimeError("ibp_fma_peak failed (%d)" % rc)
+        out[name + "_tflops"] = fl.value / (ms.value * 1e-3) / 1e12
+    return out
+
+
+def hbm_kernels():
+    """Bandwidth of the HBM-bound kernels of the step loop at large sizes, in algorithmic
+    bytes (FP64 reads + writes each kernel must make) per second, beside a device copy."""
+    import torch
+    lib = _load_pkg()._lib.load()
+    st = torch.cuda.current_stream()
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    h = st.cuda_stream
+    npt = 1 << 27  # 1 GiB of FP64 per array
+    a = torch.rand(npt, dtype=torch.float64, device="cuda")
+    b = torch.rand(npt, dtype=torch.float64, device="cuda")
+    c = torch.empty_like(a)
+    res = {}
+    sec = timed(lambda: c.copy_(a))
+    copy_gbs = 16 * npt / sec / 1e9
+    res["device_copy_GBps"] = copy_gbs
+
+    def rec(name, nbytes, sec, what):
+        gbs = nbytes / sec / 1e9
+        res[name] = {"GBps": gbs, "frac_of_copy": gbs / copy_gbs, "frac_of_8TBps": gbs / 8000.0,
+                     "ms": sec * 1e3, "bytes_alg": nbytes, "what": what}
+
+    def chk(rc):
+        if rc != 0:
+            raise RuntimeError(_load_pkg()._lib.last_error())
+
+    sec = timed(lambda: chk(lib.ib_axpy(c.data_ptr(), a.data_ptr(), 0.5, npt, h)))
+    rec("ib_axpy", 24 * npt, sec, "y += a x, 2^27 points (24 B/point)")
+    sec = timed(lambda: chk(lib.ib_df_combine(a.data_ptr(), b.data_ptr(), c.data_ptr(), 1e-3, 1e-2, npt, h)))
+    rec("ib_df_combine", 24 * npt, sec, "adv + dt q / kn, 2^27 points (24 B/point)")
+    nx, nv = npt // 1024, 1024
+    v1 = torch.linspace(-8, 8, 32, dtype=torch.float64, device="cuda").repeat_interleave(32)
+    sec = timed(lambda: chk(lib.ib_tvd1d(a.data_ptr(), c.data_ptr(), nx, nv, v1.data_ptr(), 0.01, 2, 0,
+                                         1, 1, 32, h)))
+    rec("ib_tvd1d", 16 * npt, sec, "2nd-order minmod TVD step, Nx = %d cells x 32^2 (16 B/point)" % nx)
+    nb = npt // 32768
+    mom = torch.empty((nb, 5), dtype=torch.float64, device="cuda")
+    sec = timed(lambda: chk(lib.ib_moments(a.data_ptr(), mom.data_ptr(), nb, 3, 32, 8.0, h)))
+    rec("ib_moments", 8 * npt, sec, "mass/momentum/energy of %d cells of 32^3 (8 B/point)" % nb)
+    sec = timed(lambda: chk(lib.ib_project_conservative(c.data_ptr(), nb, 3, 32, 8.0, h)))
+    rec("ib_project_conservative", 16 * npt, sec, "in-place projection of %d cells of 32^3 (16 B/point)" % nb)
+    del a, b, c
+    return res
+
+
 def uneven_requests(nprob, precision, seed=9):
     """The uneven batch of homogeneous cooling problems.
 
@@ -393,6 +468,7 @@ def main():
                     help="cells of the 2D 32^2 side measurement (0 = skip)")
     ap.add_argument("--uneven-problems", type=int, default=256,
                     help="problems of the uneven homogeneous batch (0 = skip; 4096 = full size)")
+    ap.add_argument("--no-hbm", action="store_true", help="skip the HBM-bound kernel measurements")
     ap.add_argument("--no-uneven-autotune", action="store_true",
                     help="run the FP32 uneven batch with the default launch shapes")
     args = ap.parse_args()
@@ -515,6 +591,12 @@ def main():
         out["fp64_plain_cell_evals_per_s"] = args.cells / res["fp64_s_per_step"]
         out["fp32_over_fp64"] = res["fp64_s_per_step"] / t32
         out["fp32_vs_fp64_gpu_rel_linf_max"] = res["fp32_vs_fp64_gpu_rel_linf_max"]
+    pk = valu_peak()
+    out["roofline"]["measured_valu_fma_peak_tflops"] = pk["fp32_tflops"]
+    out["roofline"]["frac_of_measured_valu_peak"] = achieved / pk["fp32_tflops"]
+    out["valu_fma_peak"] = dict(pk, note="FMA-loop probe, no packed FP32 (the library's build flags)")
+    if not args.no_hbm:
+        out["hbm_kernels"] = hbm_kernels()
     if args.cells_2d > 0:
         out["shape_2d_32"] = run_2d(args.cells_2d, args.steps, args.warmup, not args.no_fp64)
     if args.uneven_problems > 0:

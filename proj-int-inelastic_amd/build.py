@@ -1,4 +1,9 @@
-"""Build the gfx950 HIP library ``libinelastic_hip.so`` in-tree with hipcc."""
+"""Build the gfx950 HIP library ``libinelastic_hip.so`` in-tree with hipcc.
+
+Also builds ``libinelastic_probe.so`` from ``probe/`` (a vector-ALU FMA
+throughput probe the benchmark reports beside the kernel rooflines; not part
+of the collision library).
+"""
 import concurrent.futures as cf
 import os
 import pathlib
@@ -8,6 +13,7 @@ import subprocess
 HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libinelastic_hip.so"
+PROBE_OUT = HERE / "libinelastic_probe.so"
 BUILD = HERE / "_build"
 
 COMMON = [
@@ -50,6 +56,13 @@ def build(verbose=False, jobs=8):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(tmp, OUT)
+    tmp = PROBE_OUT.with_suffix(".so.tmp")
+    cmd = [hipcc, "-shared", "-o", str(tmp)] + [str(s) for s in sorted((HERE / "probe").glob("*.hip"))] \
+        + COMMON
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("probe build failed:\n" + r.stderr[-4000:])
+    os.replace(tmp, PROBE_OUT)
     return OUT
 
 

@@ -54,40 +54,64 @@ __device__ __forceinline__ double limited(double wm, double w0, double wp, doubl
 // f, df: (outer, nx, nv) with the stencil along the middle axis; the speed of
 // column v is v1[v % vmod].  accumulate: df += increment (the second sweep of a
 // dimension-split step) instead of df = increment.
-__global__ void k_tvd1d(const double* __restrict__ f, double* __restrict__ df, long long outer,
-                        int nx, long long nv, const double* __restrict__ v1, long long vmod,
-                        double dtdx, int order, int limiter, int bc_lo, int bc_hi,
-                        int accumulate) {
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= outer * nx * nv) return;
-  const long long o = gid / ((long long)nx * nv);
-  const int i = (int)((gid / nv) % nx);
-  const long long v = gid % nv;
-  const int J = i + 2;
-  const double* fo = f + o * nx * nv;
-  double F[5];
+//
+// A thread owns TVD_R consecutive cells i0 .. i0 + R - 1 of one column (o, v):
+// it loads the R + 4 rows of the ghost-extended column once, forms the R + 3
+// waves and the R + 1 limited waves of its interfaces (each interface's
+// division done once), then writes R increments.  Workgroups cover 256
+// columns of one (o, cell block); the block coordinates come from the
+// workgroup index, so the per-thread index arithmetic is 32-bit.
+constexpr int TVD_R = 4;
+constexpr int TVD_THREADS = 256;
+
+__global__ __launch_bounds__(TVD_THREADS) void k_tvd1d(
+    const double* __restrict__ f, double* __restrict__ df, int nx, long long nv,
+    const double* __restrict__ v1, long long vmod, double dtdx, int order, int limiter,
+    int bc_lo, int bc_hi, int accumulate, unsigned vchunks, unsigned ichunks) {
+  const unsigned b = blockIdx.x;
+  const unsigned vc = b % vchunks, rest = b / vchunks;
+  const int i0 = (int)(rest % ichunks) * TVD_R;
+  const long long o = rest / ichunks;
+  const long long v = (long long)vc * TVD_THREADS + threadIdx.x;
+  if (v >= nv) return;
+  const int cnt = min(TVD_R, nx - i0);
+  const double* fo = f + o * nx * nv + v;
+  double* dfo = df + o * nx * nv + v;
+  double F[TVD_R + 4];
 #pragma unroll
-  for (int t = 0; t < 5; ++t) F[t] = fo[ghost_row(J - 2 + t, nx, bc_lo, bc_hi) * nv + v];
-  const double s = v1[v % vmod];
-  const double wJm2 = F[1] - F[0];
-  const double wJm1 = F[2] - F[1];
-  const double wJ = F[3] - F[2];
-  const double wJp1 = F[4] - F[3];
-  const double amdf = fmin(s, 0.0) * wJ;
-  const double apdf = fmax(s, 0.0) * wJm1;
-  double d = (-dtdx) * (amdf + apdf);
-  if (order == 2) {
-    const double spos = (s > 0.0) ? 1.0 : 0.0;
-    const double lm1 = limited(wJm2, wJm1, wJ, spos, limiter);
-    const double l0 = limited(wJm1, wJ, wJp1, spos, limiter);
-    const double sabs = fabs(s);
-    const double om = 1.0 - sabs * dtdx;
-    const double c = (0.5 * sabs) * om;
-    const double F0 = c * lm1;
-    const double F1 = c * l0;
-    d = d - dtdx * (F1 - F0);
+  for (int t = 0; t < TVD_R + 4; ++t) {
+    const int tt = min(t, cnt + 3);  // rows past the column's end are not used
+    F[t] = fo[ghost_row(i0 + tt, nx, bc_lo, bc_hi) * nv];
   }
-  df[gid] = accumulate ? df[gid] + d : d;
+  const double s = v1[v % vmod];
+  double w[TVD_R + 3];
+#pragma unroll
+  for (int t = 0; t < TVD_R + 3; ++t) w[t] = F[t + 1] - F[t];
+  const double spos = (s > 0.0) ? 1.0 : 0.0;
+  double lim[TVD_R + 1];
+  if (order == 2) {
+#pragma unroll
+    for (int u = 0; u < TVD_R + 1; ++u) lim[u] = limited(w[u], w[u + 1], w[u + 2], spos, limiter);
+  }
+  const double sabs = fabs(s);
+  const double om = 1.0 - sabs * dtdx;
+  const double c = (0.5 * sabs) * om;
+#pragma unroll
+  for (int r = 0; r < TVD_R; ++r) {
+    if (r < cnt) {
+      // cell i0 + r: waves w[r] .. w[r + 3] are W_{J-2} .. W_{J+1} (J = i + 2)
+      const double amdf = fmin(s, 0.0) * w[r + 2];
+      const double apdf = fmax(s, 0.0) * w[r + 1];
+      double d = (-dtdx) * (amdf + apdf);
+      if (order == 2) {
+        const double F0 = c * lim[r];
+        const double F1 = c * lim[r + 1];
+        d = d - dtdx * (F1 - F0);
+      }
+      double* out = dfo + (long long)(i0 + r) * nv;
+      *out = accumulate ? *out + d : d;
+    }
+  }
 }
 
 __global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, double a, long long n) {
@@ -221,9 +245,17 @@ static VGrid vgrid(int dim, int n) {
 int launch_tvd1d(const double* f, double* df, long long outer, int nx, long long nv,
                  const double* v1, long long vmod, double dtdx, int order, int limiter, int bc_lo,
                  int bc_hi, int accumulate, hipStream_t st) {
-  const long long total = outer * nx * nv;
-  hipLaunchKernelGGL(k_tvd1d, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, f, df,
-                     outer, nx, nv, v1, vmod, dtdx, order, limiter, bc_lo, bc_hi, accumulate);
+  if (outer <= 0 || nx <= 0 || nv <= 0) return IB_OK;
+  const long long vchunks = (nv + TVD_THREADS - 1) / TVD_THREADS;
+  const long long ichunks = (nx + TVD_R - 1) / TVD_R;
+  const long long blocks = vchunks * ichunks * outer;
+  if (blocks > 0x7fffffffLL) {
+    set_error("transport grid too large (more than 2^31 workgroups)");
+    return IB_ERR_ARG;
+  }
+  hipLaunchKernelGGL(k_tvd1d, dim3((unsigned)blocks), dim3(TVD_THREADS), 0, st, f, df, nx, nv, v1,
+                     vmod, dtdx, order, limiter, bc_lo, bc_hi, accumulate, (unsigned)vchunks,
+                     (unsigned)ichunks);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }

@@ -130,3 +130,21 @@ def test_python_layer_rejects_before_the_device(pkg):
     s.bc_upper[0] = 5
     with pytest.raises(NotImplementedError):
         s._bc_codes(0)
+
+
+def test_probe_library_rejects_bad_arguments():
+    """The FMA-throughput probe the benchmark loads validates before any HIP call."""
+    import pathlib
+    so = pathlib.Path(__file__).resolve().parents[1] / "proj-int-inelastic_amd" / "libinelastic_probe.so"
+    if not so.exists():
+        pytest.skip("probe library not built")
+    lib = ctypes.CDLL(str(so))
+    fn = lib.ibp_fma_peak
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    fl, ms = ctypes.c_double(), ctypes.c_double()
+    assert fn(0, 0, 1, 1, ctypes.byref(fl), ctypes.byref(ms)) == -1
+    assert fn(0, 16, 0, 1, ctypes.byref(fl), ctypes.byref(ms)) == -1
+    assert fn(2, 16, 1, 1, ctypes.byref(fl), ctypes.byref(ms)) == -1
+    assert fn(0, 16, 1, 1, None, ctypes.byref(ms)) == -1

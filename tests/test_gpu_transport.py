@@ -50,6 +50,29 @@ def test_tvd_oracle_config_grid(pkg, need_gpu, limiter, bc):
     assert np.array_equal(df, ref)
 
 
+@pytest.mark.parametrize("nx", [1, 2, 3, 5, 14, 37])
+@pytest.mark.parametrize("bc", [1, 2])
+@pytest.mark.parametrize("order", [1, 2])
+def test_tvd_oracle_ragged(pkg, need_gpu, nx, bc, order):
+    """Cell counts that are not a multiple of the cells a thread updates, tiny grids and
+    a velocity count that is not a multiple of the workgroup width."""
+    if bc == 2 and nx < 2:
+        pytest.skip("periodic boundaries need nx >= 2")
+    lib = pkg._lib.load()
+    N = min(18, nx + 2) if order == 2 else 18  # order 2 needs nx + 2 >= N (reference broadcast)
+    rng = np.random.default_rng(nx * 7 + bc + 100 * order)
+    v = np.linspace(-3, 3, N)
+    v1 = np.repeat(v[:, None], N, axis=1)
+    f = rng.random((nx, N, N))
+    f[nx // 2:] = 0.5  # flat region: zero waves take the masked branch
+    dx = 1.0 / nx
+    dt = 0.3 * dx / 3
+    rc, df = _tvd(lib, f, v1, dt / dx, order, 0, bc, bc)
+    assert rc == 0
+    ref = T.df_advection(f, v1, dt, dx, bc, bc, order, T.minmod)
+    assert np.array_equal(df, ref)
+
+
 def test_tvd_errors(pkg, need_gpu):
     lib = pkg._lib.load()
     f = np.zeros((4, 8, 8))
