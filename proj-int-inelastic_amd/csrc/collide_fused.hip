@@ -155,7 +155,6 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   const double crad_j = A.crad[ps * A.nr + j];
   const double rpl = M_PI * A.rj[j] / A.L;
   const int p = tid / N, l = tid % N;                 // plane-pass role
-  const int xy = tid / 16, xz = 2 * (tid % 16);        // x-pass role: pencils (xy, xz), (xy, xz+1)
   const cx<float>* fh = A.fh + (A.cell0 + cell) * 17 * PLANE;
   const float* fc = A.f32 + (A.cell0 + cell) * NPTS;
   const int kyf = kfreq(l, N);
@@ -260,41 +259,63 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
 #pragma unroll
     for (int k = 0; k < N; ++k) W[p][k][l] = v[k];
     __syncthreads();
-    // C: along x, two real pencils (xy, xz), (xy, xz + 1) as one complex FFT
+    // C: along x, two real pencils (xy, xz), (xy, xz + 16) as one complex FFT
+    // (16 lanes of a row touch 16 consecutive complex values: no LDS bank conflicts)
     {
+      // roles from an opaque copy of tid, so the pencils' LDS addresses are formed
+      // here each pair instead of occupying registers across the pair loop
+      int tq = tid;
+      asm volatile("" : "+v"(tq));
+      const int xy = tq / 16, xz = tq % 16;
       cx<float> Z[N];
       {
-        const cx<float> a0 = W[0][xy][xz], b0 = W[0][xy][xz + 1];
+        const cx<float> a0 = W[0][xy][xz], b0 = W[0][xy][xz + 16];
         Z[0] = cx<float>{a0.re, b0.re};
         Z[N / 2] = cx<float>{a0.im, b0.im};
       }
 #pragma unroll
       for (int i = 1; i < N / 2; ++i) {
         const int k = bitrev<N / 2>(i);
-        const cx<float> a = W[k][xy][xz], bb = W[k][xy][xz + 1];
+        const cx<float> a = W[k][xy][xz], bb = W[k][xy][xz + 16];
         Z[k] = cx<float>{a.re - bb.im, a.im + bb.re};
         Z[N - k] = cx<float>{a.re + bb.im, bb.re - a.im};
       }
-      fft_reg<N, +1>(Z);
       // 32-bit per-thread offset, made opaque per pair so the 32 addresses are
       // formed next to their loads instead of being hoisted out of the loop
       int toff = xy * N + xz;
       asm volatile("" : "+v"(toff));
+      // f along the two pencils, in two halves of 16 x: the first half is loaded
+      // before the inverse transform (its latency hides behind the FFT), the
+      // second before the first is used
+      float fa[N], fb[N];
 #pragma unroll
-      for (int x = 0; x < N; ++x) {
-        const float fa = fc[x * (int)PLANE + toff], fb = fc[x * (int)PLANE + toff + 1];
-        Z[x] = cx<float>{Z[x].re * fa, Z[x].im * fb};
-        if ((x & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      for (int x = 0; x < N / 2; ++x) {
+        fa[x] = fc[x * (int)PLANE + toff];
+        fb[x] = fc[x * (int)PLANE + toff + 16];
       }
+      __builtin_amdgcn_sched_barrier(0);
+      fft_reg<N, +1>(Z);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int x = N / 2; x < N; ++x) {
+        fa[x] = fc[x * (int)PLANE + toff];
+        fb[x] = fc[x * (int)PLANE + toff + 16];
+      }
+#pragma unroll
+      for (int x = 0; x < N / 2; ++x) Z[x] = cx<float>{Z[x].re * fa[x], Z[x].im * fb[x]};
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int x = N / 2; x < N; ++x) Z[x] = cx<float>{Z[x].re * fa[x], Z[x].im * fb[x]};
+      __builtin_amdgcn_sched_barrier(0);
       fft_reg<N, -1>(Z);
       W[0][xy][xz] = cx<float>{Z[0].re, Z[N / 2].re};
-      W[0][xy][xz + 1] = cx<float>{Z[0].im, Z[N / 2].im};
+      W[0][xy][xz + 16] = cx<float>{Z[0].im, Z[N / 2].im};
 #pragma unroll
       for (int k = 1; k < N / 2; ++k) {
         const cx<float> zk = Z[k], zm = Z[N - k];
         // planes 1..15 carry twice their value; phase E folds the 1/2 into its weight
         W[k][xy][xz] = cx<float>{zk.re + zm.re, zk.im - zm.im};
-        W[k][xy][xz + 1] = cx<float>{zk.im + zm.im, zm.re - zk.re};
+        W[k][xy][xz + 16] = cx<float>{zk.im + zm.im, zm.re - zk.re};
       }
     }
     // side work: the packed plane of the previous pair, the phase tables two pairs
