@@ -1,18 +1,30 @@
-// Fused FP32 form of Q(f, f) for 2D grids (N = 8, 16, 32).
+// Fused FP32 C2C forms of Q(f, f): 2D grids (N = 16, 32) and the 3D 16^3 grid.
 //
 // Same operator as reference src/collision/inelastic_collisions.py:208-227 with
-// C2C transforms (the reference's own semantics: the N x N complex work array
-// of a quadrature pair fits in LDS, so no real-data reduction is needed).  A
-// workgroup serves one cell and a strided subset of its (j, m) quadrature
-// pairs with G = THREADS / N concurrent pairs, one N-thread group per pair:
+// C2C transforms (the reference's own semantics: the complex work array of a
+// quadrature pair fits in LDS, so no real-data reduction is needed, and grids
+// this coarse can carry Nyquist content the real-data form would symmetrise).
+//
+// 2D: a workgroup serves one cell and a strided subset of its (j, m)
+// quadrature pairs with G = THREADS / N concurrent pairs, one N-thread group
+// per pair:
 //   A  row pencils: E * fhat, inverse along axis 1
 //   B  column pencils: inverse along axis 0, * f, forward along axis 0
 //   C  row pencils: forward along axis 1, * G, sum over the group's pairs in
 //      registers
 // fhat / n and f of the cell are staged in LDS once per workgroup.  The
-// groups' sums are added in LDS, each workgroup writes an FP32 partial; the
-// FP64 sum over workgroups, the final inverse transform, the loss term and the
-// Laplacian run in FP64 (collide_plain.hip), as does the prologue fhat = fft(f).
+// groups' sums are added in LDS, each workgroup writes an FP32 partial.
+//
+// 3D 16^3: one 256-thread workgroup per (cell, radial node j) loops over the
+// M angles with the 16^3 complex work array in LDS; a thread owns one pencil
+// in each of the five passes (E * fhat and inverse z; inverse y; inverse x,
+// * f, forward x; forward y; forward z, * G and the sum over angles).  The
+// thread's fhat pencil and f pencil stay in registers for the whole loop, and
+// sp(a_j r_j |k|) and the radial weight are applied once at the end.
+//
+// In both forms the FP64 sum over workgroups (radial nodes), the final inverse
+// transform, the loss term and the Laplacian run in FP64 (collide_plain.hip),
+// as does the prologue fhat = fft(f).
 #include <algorithm>
 
 #include "common.h"
@@ -216,6 +228,203 @@ __global__ void k_f2_reduce(const cx<float>* __restrict__ part, cx<double>* __re
   acc64[gid] = cx<double>{re, im};
 }
 
+namespace c3 {
+constexpr int N = 16;
+constexpr int PAD = N + 1;
+constexpr int THREADS = N * N;
+constexpr long long NPTS = (long long)N * N * N;
+}  // namespace c3
+
+struct C3Args {
+  const cx<float>* fh;  // [cell][N][N][N], fhat / n
+  const float* f32;     // [cell][N][N][N]
+  cx<float>* part;      // [cell][nr][N][N][N]
+  const int* pidx;
+  long long batch;
+  const double* sigma;
+  const double* rj;
+  const double* wsig;
+  const double* crad;
+  const double* a;
+  const float* spg;
+  int nr, m, nq;
+  double L;
+};
+
+// Pair schedule (4 workgroup barriers per quadrature pair); W[x][y][z] rows are
+// padded to N + 1, so every pass reads and writes 16 lanes of distinct banks.
+__global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
+  using namespace c3;
+  __shared__ cx<float> W[N][N][PAD];
+  __shared__ cx<float> tE[2][3][N];
+  __shared__ cx<float> tG[2][3][N];
+  __shared__ float sp[3 * (N / 2) * (N / 2) + 1];
+  const int tid = threadIdx.x;
+  const long long id = blockIdx.x;
+  const int c8 = (int)(id % 8);  // the 8 * nr workgroups of 8 cells share one XCD per cell
+  const int j = (int)((id / 8) % A.nr);
+  const long long cell = (id / (8LL * A.nr)) * 8 + c8;
+  if (cell >= A.batch) return;  // uniform over the workgroup
+  const int ps = A.pidx ? A.pidx[cell] : 0;
+  for (int q = tid; q < A.nq; q += THREADS) sp[q] = A.spg[((long long)ps * A.nr + j) * A.nq + q];
+  const double aj = A.a[ps * A.nr + j] * A.rj[j];
+  const double rpl = M_PI * A.rj[j] / A.L;
+  // pencil role: the two fixed indices (r0, r1) of the thread's pencil in every pass
+  const int r0 = tid / N, r1 = tid % N;
+  cx<float> fhv[N];  // fhat along z at (kx, ky) = (r0, r1)
+  float fv[N];       // f along x at (y, z) = (r0, r1)
+  {
+    const cx<float>* fh = A.fh + cell * NPTS + (long long)(r0 * N + r1) * N;
+    const float* fc = A.f32 + cell * NPTS + r0 * N + r1;
+#pragma unroll
+    for (int k = 0; k < N; ++k) fhv[k] = fh[k];
+#pragma unroll
+    for (int x = 0; x < N; ++x) fv[x] = fc[x * N * N];
+  }
+  cx<float> acc[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = cx<float>{0.f, 0.f};
+  // phase factors of pair mm along each axis (the separable factors of
+  // inelastic_collisions.py:97-105), FP64 sincos rounded to FP32
+  auto make_tables = [&](int mm, int b) {
+    if (tid < 6 * N) {
+      const int which = tid / (3 * N), ax = (tid % (3 * N)) / N, k = tid % N;
+      const double s = A.sigma[mm * 3 + ax];
+      const double kf = (double)kfreq(k, N);
+      const double th = which == 0 ? -(rpl * s * kf) : (aj * s * kf);
+      double sn, cs;
+      sincos(th, &sn, &cs);
+      (which ? tG : tE)[b][ax][k] = cx<float>{(float)cs, (float)sn};
+    }
+  };
+  make_tables(0, 0);
+  __syncthreads();
+  cx<float> v[N];
+  for (int mm = 0; mm < A.m; ++mm) {
+    const int b = mm & 1;
+    // A: E * fhat along z at (kx, ky) = (r0, r1), inverse along z
+    {
+      const cx<float> row = tE[b][0][r0] * tE[b][1][r1];
+#pragma unroll
+      for (int k = 0; k < N; ++k) v[k] = (row * tE[b][2][k]) * fhv[k];
+      fft_reg<N, +1>(v);
+#pragma unroll
+      for (int z = 0; z < N; ++z) W[r0][r1][z] = v[z];
+    }
+    __syncthreads();
+    // B: inverse along y at (kx, z) = (r0, r1); the next pair's phase tables (the
+    // other buffer: its last reader, the previous pair, is behind the barrier above)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int k = bitrev<N>(i);
+      v[k] = W[r0][k][r1];
+    }
+    fft_reg<N, +1>(v);
+#pragma unroll
+    for (int y = 0; y < N; ++y) W[r0][y][r1] = v[y];
+    if (mm + 1 < A.m) make_tables(mm + 1, b ^ 1);
+    __syncthreads();
+    // C: along x at (y, z) = (r0, r1): inverse, * f, forward
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int k = bitrev<N>(i);
+      v[k] = W[k][r0][r1];
+    }
+    fft_reg<N, +1>(v);
+#pragma unroll
+    for (int x = 0; x < N; ++x) v[x] = fv[x] * v[x];
+    fft_reg<N, -1>(v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) W[k][r0][r1] = v[k];
+    __syncthreads();
+    // D: forward along y at (kx, z) = (r0, r1)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int k = bitrev<N>(i);
+      v[k] = W[r0][k][r1];
+    }
+    fft_reg<N, -1>(v);
+#pragma unroll
+    for (int k = 0; k < N; ++k) W[r0][k][r1] = v[k];
+    __syncthreads();
+    // E: forward along z at (kx, ky) = (r0, r1), G * (.) and the sum over angles
+    // (the thread's own row: the next pair's phase A may overwrite it right after)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int k = bitrev<N>(i);
+      v[k] = W[r0][r1][k];
+    }
+    fft_reg<N, -1>(v);
+    const cx<float> grow = (float)A.wsig[mm] * (tG[b][0][r0] * tG[b][1][r1]);
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] = acc[k] + (grow * tG[b][2][k]) * v[k];
+  }
+  // radial weight and sp(a_j r_j |k|), then this (cell, j)'s partial spectrum
+  const float cj = (float)A.crad[ps * A.nr + j];
+  const int kx = kfreq(r0, N), ky = kfreq(r1, N);
+  cx<float>* out = A.part + ((long long)cell * A.nr + j) * NPTS + (long long)(r0 * N + r1) * N;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const int kz = kfreq(k, N);
+    out[k] = (cj * sp[kx * kx + ky * ky + kz * kz]) * acc[k];
+  }
+}
+
+static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                       int with_loss, double hb, hipStream_t st) {
+  using namespace c3;
+  const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 1024);
+  const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
+  int rc;
+  if ((rc = P->fhat.ensure(sub * b64))) return rc;
+  if ((rc = P->acc.ensure(sub * b64))) return rc;
+  if ((rc = P->work.ensure(2 * sub * b64))) return rc;
+  const size_t part_b = (size_t)sub * P->nr * NPTS * sizeof(cx<float>);
+  const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<float>);
+  if ((rc = P->aux.ensure(part_b + fh_b + (size_t)sub * NPTS * sizeof(float)))) return rc;
+  cx<float>* part = (cx<float>*)P->aux.ptr;
+  cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + part_b);
+  float* f32 = (float*)((char*)fh32 + fh_b);
+  for (long long c0 = 0; c0 < batch; c0 += sub) {
+    const long long nb = std::min(sub, batch - c0);
+    const double* fc = f + c0 * NPTS;
+    const int* pc = pidx ? pidx + c0 : nullptr;
+    if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
+    const long long tot = nb * NPTS;
+    hipLaunchKernelGGL(k_f2_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       (const cx<double>*)P->fhat.ptr, fc, fh32, f32, tot, 1.0 / (double)NPTS);
+    IB_HIP(hipGetLastError());
+    C3Args A{};
+    A.fh = fh32;
+    A.f32 = f32;
+    A.part = part;
+    A.pidx = pc;
+    A.batch = nb;
+    A.sigma = P->d_sigma;
+    A.rj = P->d_rj;
+    A.wsig = P->d_wsig;
+    A.crad = P->d_crad;
+    A.a = P->d_a;
+    A.spg = (const float*)P->d_spg;
+    A.nr = P->nr;
+    A.m = P->m;
+    A.nq = P->nq;
+    A.L = P->L;
+    const long long groups = (nb + 7) / 8;
+    if ((rc = P->timed_begin(st))) return rc;
+    hipLaunchKernelGGL(k_fusedc3, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
+    IB_HIP(hipGetLastError());
+    P->timed_end(st, nb);
+    hipLaunchKernelGGL(k_f2_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
+                       (cx<double>*)P->acc.ptr, nb, P->nr, NPTS);
+    IB_HIP(hipGetLastError());
+    if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
+                                   with_loss, hb, pc)))
+      return rc;
+  }
+  return IB_OK;
+}
+
 template <int N>
 static int run_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                       int with_loss, double hb, hipStream_t st) {
@@ -283,6 +492,11 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
 int collide_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                    int with_loss, double hb, hipStream_t st, int* handled) {
   *handled = 1;
+  if (P->dim == 3) {
+    if (P->n == 16) return run_fusedc3(P, f, q, batch, pidx, with_loss, hb, st);
+    *handled = 0;
+    return IB_OK;
+  }
   switch (P->n) {
     case 16: return run_fused2<16>(P, f, q, batch, pidx, with_loss, hb, st);
     case 32: return run_fused2<32>(P, f, q, batch, pidx, with_loss, hb, st);

@@ -134,8 +134,10 @@ class _Class:
             a.start = start
             start += a.cells
             act.append(a)
-        for a, f0 in new:
-            parts.append(dev.to_device(f0.reshape((a.cells,) + self.vshape)))
+        if new:  # one host-to-device copy for all admitted problems
+            parts.append(dev.to_device(np.concatenate(
+                [f0.reshape((a.cells,) + self.vshape) for a, f0 in new])))
+        for a, _ in new:
             a.start = start
             start += a.cells
             act.append(a)
@@ -164,17 +166,33 @@ class _Class:
         lib = _lib.load()
         st = dev.stream_handle()
         out = t.empty_like(f)
+        run = None  # [first cell, cells, kn] of adjacent homogeneous problems with one kn
+
+        def combine(c0, nc, adv, kn):
+            sl = slice(c0, c0 + nc)
+            _lib.check(lib.ib_df_combine(dev.ptr(adv), q[sl].data_ptr(), out[sl].data_ptr(), float(h),
+                                         float(kn), out[sl].numel(), st), "ib_df_combine")
+
         for a in self.active:
-            sl = slice(a.start, a.start + a.cells)
-            adv = None
             if a.req.xlen > 0:
+                if run is not None:
+                    combine(run[0], run[1], None, run[2])
+                    run = None
+                sl = slice(a.start, a.start + a.cells)
                 adv = t.empty_like(f[sl])
                 dx = a.req.xlen / a.cells
                 _lib.check(lib.ib_tvd1d(f[sl].data_ptr(), adv.data_ptr(), a.cells, f[0].numel(),
                                         self.v1.data_ptr(), float(h / dx), 2, 0, 1, 1,
                                         self.proto.n, st), "ib_tvd1d")
-            _lib.check(lib.ib_df_combine(dev.ptr(adv), q[sl].data_ptr(), out[sl].data_ptr(), float(h),
-                                         float(a.req.kn), out[sl].numel(), st), "ib_df_combine")
+                combine(a.start, a.cells, adv, a.req.kn)
+            elif run is not None and run[2] == a.req.kn and run[0] + run[1] == a.start:
+                run[1] += a.cells
+            else:
+                if run is not None:
+                    combine(run[0], run[1], None, run[2])
+                run = [a.start, a.cells, a.req.kn]
+        if run is not None:
+            combine(run[0], run[1], None, run[2])
         return out
 
 
@@ -282,18 +300,21 @@ class Server:
         """Admit arrived requests in arrival order while memory allows."""
         admitted = {}
         waiting = []
+        cells = {}  # class key -> resident + newly admitted cells
         for r in sorted(self.queue, key=lambda r: r.arrival):
             if r.arrival > now:
                 waiting.append(r)
                 continue
-            cls = self.classes.get(r.class_key())
-            cells_now = sum(a.cells for a in cls.active) if cls else 0
-            cells_new = sum(a.cells for a, _ in admitted.get(r.class_key(), []))
+            key = r.class_key()
+            if key not in cells:
+                cls = self.classes.get(key)
+                cells[key] = sum(a.cells for a in cls.active) if cls else 0
             fits_mem = self.used + r.nbytes() <= self.budget
-            fits_cls = self.max_cells is None or cells_now + cells_new + r.cells <= self.max_cells
+            fits_cls = self.max_cells is None or cells[key] + r.cells <= self.max_cells
             if not (fits_mem and fits_cls):
                 waiting.append(r)
                 continue
+            cells[key] += r.cells
             n = int(round(r.t_final / r.dt))
             if abs(n * r.dt - r.t_final) > 1e-5 * r.t_final:
                 raise Exception("dt does not divide (tend-tstart) and dt is fixed!")
@@ -336,8 +357,13 @@ class Server:
                 done = [a for a in cls.active if a.steps_left <= 0]
                 if not done:
                     continue
+                # one device-to-host copy per class when several problems finish
+                host = cls.f.cpu().numpy() if len(done) > 4 else None
                 for a in done:
-                    fh = cls.f[a.start:a.start + a.cells].cpu().numpy().reshape(a.req.f0.shape)
+                    if host is not None:
+                        fh = host[a.start:a.start + a.cells].reshape(a.req.f0.shape).copy()
+                    else:
+                        fh = cls.f[a.start:a.start + a.cells].cpu().numpy().reshape(a.req.f0.shape)
                     self.results[a.req.rid] = Result(a.req.rid, fh, a.t, a.req.arrival,
                                                      a.admitted_at, done_t)
                     self.used -= a.req.nbytes()

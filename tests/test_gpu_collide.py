@@ -270,6 +270,32 @@ def test_fused_fp32_2d_batch(pkg, need_gpu, N, nr, M, s, with_loss, hb):
     assert worst <= TOL32, worst
 
 
+@pytest.mark.parametrize("with_loss,hb", [(True, 0.0), (False, 0.05)])
+def test_fused_fp32_3d16_batch(pkg, need_gpu, with_loss, hb):
+    """Fused FP32 C2C form of the 3D 16^3 grid (one workgroup per (cell, radial node)):
+    13 cells with per-cell e and gamma and grid-scale noise, in sub-batches of 5 cells
+    (a partial group of 8 in each), against the FP64 oracle cell by cell."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    d, N, nr, M, s = 3, 16, 16, 38, 3.0
+    cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 0.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision="fp32")
+    coll.set_tuning(fused_cells=5)
+    es = np.linspace(0.2, 1.0, 13)
+    gs = np.array([0.0, 1.0] * 6 + [1.0])
+    f = _data(d, N, sm.L, 13, seed=316)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=with_loss, heat_bath=hb, e=es,
+                     gamma=gs).cpu().numpy()
+    worst = 0.0
+    for c in range(13):
+        op = _oracle(d, N, nr, M, s, es[c], es[c], gs[c])
+        ref = op(f[c], heat_bath=hb) if not with_loss else op.collide(f[c], with_loss=True)
+        worst = max(worst, _err(q[c], ref))
+    print("fused FP32 3D16 worst rel Linf", worst)
+    assert worst <= TOL32, worst
+
+
 def test_fused_fp32_2d_many_parameter_sets(pkg, need_gpu):
     """More parameter sets than the precomputed phase tables hold (70 > 64): the 2D fused
     kernel evaluates its gain phase factors itself; checked against the oracle."""
@@ -289,7 +315,8 @@ def test_fused_fp32_2d_many_parameter_sets(pkg, need_gpu):
     assert worst <= TOL32, worst
 
 
-@pytest.mark.parametrize("d,N,nr,M,s", [(3, 32, 8, 14, 3.624654714576), (2, 32, 16, 8, 3.0)])
+@pytest.mark.parametrize("d,N,nr,M,s", [(3, 32, 8, 14, 3.624654714576), (2, 32, 16, 8, 3.0),
+                                       (3, 16, 16, 38, 3.0)])
 def test_fused_fp32_batch_invariance_full_size(pkg, need_gpu, d, N, nr, M, s):
     """Size-independent property at batch sizes beyond the oracle's reach: 300 cells made of
     10 distinct cells repeated (several sub-batches, every XCD group position, per-cell e) give
