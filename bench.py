@@ -388,26 +388,39 @@ def uneven_run(precision, nprob, autotune=False):
     return out, res
 
 
-def run_2d(nb, steps, warmup, with_fp64):
-    """Side measurement on the 2D 32^2 shape of the homogeneous-cooling and 1D x 2D runs
-    (Nr = 32, M = 8, hard spheres, e = 0.9): nb cells, gain + loss, FP32 fused form
-    (k_fused2) and plain FP64 form; roofline of k_fused2 from its live HIP-event time."""
+SIDE_SHAPES = {
+    # name: (dim, N, Nr, M, s, fused kernel, what)
+    "2d32": (2, 32, 32, 8, S3, "k_fused2",
+             "2D 32^2, Nr=32, M=8 (the homogeneous-cooling and 1D x 2D shape)"),
+    "3d16": (3, 16, 16, 38, 3.0, "k_fusedc3",
+             "3D 16^3, Nr=16, M=38 Lebedev, s=3 (the 3D shape of the uneven batch and serving mix)"),
+}
+
+
+def run_shape(name, nb, steps, warmup, with_fp64):
+    """Side measurement on one more shape: nb cells of seeded anisotropic Gaussians
+    (per-axis T ~ U[0.5, 2]), hard spheres, e = 0.9, gain + loss; the fused FP32 form and
+    the plain FP64 form; roofline of the fused kernel from its live HIP-event time."""
     import torch
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
-    n2, nr2, m2, s2 = 32, 32, 8, 3.624654714576
-    cfg = {"collision-model": {"dim": 2, "e": 0.9, "e0": 0.9, "gamma": 1.0},
-           "velocity-mesh": {"nv": n2, "s": s2, "nr": nr2}, "circle-quad-rule": {"nphi": m2}}
+    d, n, nr, m, s, kname, what = SIDE_SHAPES[name]
+    cfg = {"collision-model": {"dim": d, "e": 0.9, "e0": 0.9, "gamma": 1.0},
+           "velocity-mesh": {"nv": n, "s": s, "nr": nr}}
+    if d == 2:
+        cfg["circle-quad-rule"] = {"nphi": m}
+    else:
+        cfg["spherical-design-rule"] = {"ssrule": "lebedev", "nsphr": m}
     sm = SpectralMesh(cfg)
     rng = np.random.default_rng(5)
-    dv = 2 * sm.L / n2
-    v = -sm.L + (np.arange(n2) + 0.5) * dv
-    vx, vy = np.meshgrid(v, v, indexing="ij")
-    T = rng.uniform(0.5, 2.0, (nb, 2))
-    f = np.exp(-vx[None] ** 2 / (2 * T[:, 0, None, None]) - vy[None] ** 2 / (2 * T[:, 1, None, None]))
-    f /= f.sum(axis=(1, 2), keepdims=True) * dv * dv
+    dv = 2 * sm.L / n
+    v = -sm.L + (np.arange(n) + 0.5) * dv
+    grids = np.meshgrid(*([v] * d), indexing="ij")
+    T = rng.uniform(0.5, 2.0, (nb, d))
+    f = np.exp(-sum(grids[a][None] ** 2 / (2 * T[:, a].reshape((nb,) + (1,) * d)) for a in range(d)))
+    f /= f.sum(axis=tuple(range(1, d + 1)), keepdims=True) * dv ** d
     f_dev = torch.from_numpy(f).cuda()
-    res = {"workload": "2D 32^2, Nr=32, M=8, hard spheres, e=0.9, gain+loss, %d cells" % nb}
-    flop = flops_per_cell(n2 * n2, nr2, m2)
+    res = {"workload": "%s, hard spheres, e=0.9, gain+loss, %d cells" % (what, nb)}
+    flop = flops_per_cell(n ** d, nr, m)
     out = {}
     for prec in (("fp32", "fp64") if with_fp64 else ("fp32",)):
         c = FSInelasticVHSCollision(cfg, sm, precision=prec)
@@ -429,16 +442,16 @@ def run_2d(nb, steps, warmup, with_fp64):
             kms, kl, ku = c.get_timing()
             kavg = kms / max(kl, 1)
             ach = flop * (ku / max(kl, 1)) / (kavg * 1e-3) / 1e12
-            res["roofline_k_fused2"] = {"bound": "mfma", "compute_unit": "valu", "achieved": ach,
-                                        "peak": 157.3,
-                                        "unit": "TFLOP/s", "frac": ach / 157.3,
+            res["roofline_" + kname] = {"bound": "mfma", "compute_unit": "valu", "achieved": ach,
+                                        "peak": 157.3, "unit": "TFLOP/s", "frac": ach / 157.3,
                                         "kernel_avg_ms": kavg, "cells_per_launch": ku / max(kl, 1),
                                         "flop_per_cell_alg": flop}
     if "fp64" in out:
         a, b = out["fp32"], out["fp64"]
+        axes = tuple(range(1, d + 1))
         res["fp32_over_fp64"] = res["fp32_cell_evals_per_s"] / res["fp64_cell_evals_per_s"]
-        res["fp32_vs_fp64_rel_linf_max"] = float(((a - b).abs().amax(dim=(1, 2)) /
-                                                  b.abs().amax(dim=(1, 2))).max())
+        res["fp32_vs_fp64_rel_linf_max"] = float(((a - b).abs().amax(dim=axes) /
+                                                  b.abs().amax(dim=axes)).max())
     return res
 
 
@@ -451,9 +464,11 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--check-cells", type=int, default=2)
     ap.add_argument("--cpu-cells", type=int, default=36)
-    ap.add_argument("--mode", choices=["bench", "profile", "profile2d", "copy"], default="bench",
+    ap.add_argument("--mode", choices=["bench", "profile", "profile2d", "profile3d16", "copy"],
+                    default="bench",
                     help="profile: FP32 form only on --cells cells (for rocprofv3); "
                          "profile2d: the 2D 32^2 FP32 form only on --cells-2d cells; "
+                         "profile3d16: the 3D 16^3 FP32 form only on --cells-3d16 cells; "
                          "copy: a plain 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -466,6 +481,8 @@ def main():
     ap.add_argument("--serve-sweep-requests", type=int, default=32)
     ap.add_argument("--cells-2d", type=int, default=4096,
                     help="cells of the 2D 32^2 side measurement (0 = skip)")
+    ap.add_argument("--cells-3d16", type=int, default=2048,
+                    help="cells of the 3D 16^3 side measurement (0 = skip)")
     ap.add_argument("--uneven-problems", type=int, default=256,
                     help="problems of the uneven homogeneous batch (0 = skip; 4096 = full size)")
     ap.add_argument("--no-hbm", action="store_true", help="skip the HBM-bound kernel measurements")
@@ -493,7 +510,12 @@ def main():
                                                                       not args.no_fp64)}))
         return
     if args.mode == "profile2d":
-        print(json.dumps({"mode": "profile2d", **run_2d(args.cells_2d, args.steps, args.warmup, False)}))
+        print(json.dumps({"mode": "profile2d", **run_shape("2d32", args.cells_2d, args.steps, args.warmup,
+                                                           False)}))
+        return
+    if args.mode == "profile3d16":
+        print(json.dumps({"mode": "profile3d16", **run_shape("3d16", args.cells_3d16, args.steps,
+                                                             args.warmup, False)}))
         return
 
     cfg = cfg3()
@@ -598,7 +620,9 @@ def main():
     if not args.no_hbm:
         out["hbm_kernels"] = hbm_kernels()
     if args.cells_2d > 0:
-        out["shape_2d_32"] = run_2d(args.cells_2d, args.steps, args.warmup, not args.no_fp64)
+        out["shape_2d_32"] = run_shape("2d32", args.cells_2d, args.steps, args.warmup, not args.no_fp64)
+    if args.cells_3d16 > 0:
+        out["shape_3d_16"] = run_shape("3d16", args.cells_3d16, args.steps, args.warmup, not args.no_fp64)
     if args.uneven_problems > 0:
         u32, r32 = uneven_run("fp32", args.uneven_problems, autotune=not args.no_uneven_autotune)
         out["uneven_batch"] = {"workload": "homogeneous cooling, 2D 16^2/32^2/64^2 + 3D 16^3/32^3, "
