@@ -129,19 +129,30 @@ __global__ void k_df_combine(const double* __restrict__ adv, const double* __res
   }
 }
 
-template <int BS>
-__device__ double block_sum(double x, double* sh) {
-  const int t = threadIdx.x;
-  sh[t] = x;
+// Sums K values over the workgroup: a butterfly over the 64 lanes of each wave,
+// then the BS / 64 wave sums through LDS, added by every thread in the same order.
+template <int BS, int K>
+__device__ void block_sums(double (&x)[K], double* sh) {
+  constexpr int NW = BS / 64;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] += __shfl_xor(x[k], o, 64);
+  }
+  const int w = threadIdx.x / 64;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k * NW + w] = x[k];
+  }
   __syncthreads();
 #pragma unroll
-  for (int s = BS / 2; s > 0; s >>= 1) {
-    if (t < s) sh[t] = sh[t] + sh[t + s];
-    __syncthreads();
+  for (int k = 0; k < K; ++k) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) t += sh[k * NW + i];
+    x[k] = t;
   }
-  const double r = sh[0];
   __syncthreads();
-  return r;
 }
 
 // Grid coordinates of a cell's flattened velocity index: v_i = -L + (i + 1/2) dv
@@ -166,69 +177,124 @@ __device__ void fill_vtab(double* vtab, int n, double L) {
   __syncthreads();
 }
 
+// Streams the points of one cell through fn(x, i): 16-byte loads, MOM_UNROLL of
+// them issued before use (enough bytes in flight per wave to cover HBM latency),
+// scalar loads when the cell has an odd number of points.
+constexpr int MOM_UNROLL = 4;
+
+template <typename Fn>
+__device__ __forceinline__ void for_each_point(const double* __restrict__ fc, int npts, Fn fn) {
+  if ((npts & 1) == 0) {
+    const double2* f2 = reinterpret_cast<const double2*>(fc);
+    const int n2 = npts / 2;
+    int i2 = threadIdx.x;
+    for (; i2 + (MOM_UNROLL - 1) * MOM_THREADS < n2; i2 += MOM_UNROLL * MOM_THREADS) {
+      double2 x[MOM_UNROLL];
+#pragma unroll
+      for (int u = 0; u < MOM_UNROLL; ++u) x[u] = f2[i2 + u * MOM_THREADS];
+#pragma unroll
+      for (int u = 0; u < MOM_UNROLL; ++u) {
+        fn(x[u].x, 2 * (i2 + u * MOM_THREADS));
+        fn(x[u].y, 2 * (i2 + u * MOM_THREADS) + 1);
+      }
+    }
+    for (; i2 < n2; i2 += MOM_THREADS) {
+      const double2 x = f2[i2];
+      fn(x.x, 2 * i2);
+      fn(x.y, 2 * i2 + 1);
+    }
+  } else {
+    for (int i = threadIdx.x; i < npts; i += MOM_THREADS) fn(fc[i], i);
+  }
+}
+
 // out[c] = [rho, m_1..m_d, E] * dv^d, one workgroup per cell.
+template <int DIM>
 __global__ __launch_bounds__(MOM_THREADS) void k_moments(const double* __restrict__ f,
                                                          double* __restrict__ out, VGrid g,
                                                          double L) {
-  __shared__ double sh[MOM_THREADS];
+  __shared__ double sh[(DIM + 2) * (MOM_THREADS / 64)];
   __shared__ double vtab[256];
   fill_vtab(vtab, g.n, L);
   const long long c = blockIdx.x;
-  const double* fc = f + c * g.npts;
-  double s0 = 0, s1[3] = {0, 0, 0}, s2 = 0;
-  for (int i = threadIdx.x; i < g.npts; i += MOM_THREADS) {
-    const double x = fc[i];
-    s0 += x;
+  double s[DIM + 2];  // rho, m_1 .. m_d, sum f |v|^2
+#pragma unroll
+  for (int k = 0; k < DIM + 2; ++k) s[k] = 0.0;
+  for_each_point(f + c * g.npts, g.npts, [&](double x, int i) {
+    s[0] += x;
     double v2 = 0;
-    for (int a = 0; a < g.dim; ++a) {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
       const double va = vtab[g.index(i, a)];
-      s1[a] += x * va;
+      s[1 + a] += x * va;
       v2 += va * va;
     }
-    s2 += x * v2;
-  }
+    s[DIM + 1] += x * v2;
+  });
   const double dv = 2 * L / g.n;
   double w = 1;
-  for (int a = 0; a < g.dim; ++a) w *= dv;
-  const double r0 = block_sum<MOM_THREADS>(s0, sh);
-  double r1[3];
-  for (int a = 0; a < g.dim; ++a) r1[a] = block_sum<MOM_THREADS>(s1[a], sh);
-  const double r2 = block_sum<MOM_THREADS>(s2, sh);
+  for (int a = 0; a < DIM; ++a) w *= dv;
+  block_sums<MOM_THREADS>(s, sh);
   if (threadIdx.x == 0) {
-    double* o = out + c * (g.dim + 2);
-    o[0] = r0 * w;
-    for (int a = 0; a < g.dim; ++a) o[1 + a] = r1[a] * w;
-    o[g.dim + 1] = 0.5 * r2 * w;
+    double* o = out + c * (DIM + 2);
+    for (int k = 0; k < DIM + 1; ++k) o[k] = s[k] * w;
+    o[DIM + 1] = 0.5 * s[DIM + 1] * w;
   }
 }
 
 // Q <- Q - mean(Q) - sum_a (<Q, v_a> / <v_a, v_a>) v_a, one workgroup per cell.
 // The constraint rows {1, v_a} are mutually orthogonal on the symmetric grid,
 // so the projection needs only these d + 1 reductions.
+template <int DIM>
 __global__ __launch_bounds__(MOM_THREADS) void k_project(double* __restrict__ q, VGrid g,
                                                          double L) {
-  __shared__ double sh[MOM_THREADS];
+  __shared__ double sh[(DIM + 1) * (MOM_THREADS / 64)];
   __shared__ double vtab[256];
   fill_vtab(vtab, g.n, L);
   const long long c = blockIdx.x;
   double* qc = q + c * g.npts;
-  double s0 = 0, s1[3] = {0, 0, 0};
-  for (int i = threadIdx.x; i < g.npts; i += MOM_THREADS) {
-    const double x = qc[i];
-    s0 += x;
-    for (int a = 0; a < g.dim; ++a) s1[a] += x * vtab[g.index(i, a)];
-  }
-  const double r0 = block_sum<MOM_THREADS>(s0, sh);
-  double coef[3] = {0, 0, 0};
+  double s[DIM + 1];  // sum q, sum q v_a
+#pragma unroll
+  for (int k = 0; k < DIM + 1; ++k) s[k] = 0.0;
+  for_each_point(qc, g.npts, [&](double x, int i) {
+    s[0] += x;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) s[1 + a] += x * vtab[g.index(i, a)];
+  });
+  block_sums<MOM_THREADS>(s, sh);
   double vv = 0;  // sum over the grid of v_a^2 = n^(d-1) sum_i v_i^2
   for (int i = 0; i < g.n; ++i) vv += vtab[i] * vtab[i];
   vv *= (double)(g.npts / g.n);
-  for (int a = 0; a < g.dim; ++a) coef[a] = block_sum<MOM_THREADS>(s1[a], sh) / vv;
-  const double mean = r0 / (double)g.npts;
-  for (int i = threadIdx.x; i < g.npts; i += MOM_THREADS) {
-    double x = qc[i] - mean;
-    for (int a = 0; a < g.dim; ++a) x = x - coef[a] * vtab[g.index(i, a)];
-    qc[i] = x;
+  double coef[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) coef[a] = s[1 + a] / vv;
+  const double mean = s[0] / (double)g.npts;
+  auto proj = [&](double x, int i) {
+    x = x - mean;
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) x = x - coef[a] * vtab[g.index(i, a)];
+    return x;
+  };
+  if ((g.npts & 1) == 0) {
+    double2* q2 = reinterpret_cast<double2*>(qc);
+    const int n2 = g.npts / 2;
+    int i2 = threadIdx.x;
+    for (; i2 + (MOM_UNROLL - 1) * MOM_THREADS < n2; i2 += MOM_UNROLL * MOM_THREADS) {
+      double2 x[MOM_UNROLL];
+#pragma unroll
+      for (int u = 0; u < MOM_UNROLL; ++u) x[u] = q2[i2 + u * MOM_THREADS];
+#pragma unroll
+      for (int u = 0; u < MOM_UNROLL; ++u) {
+        const int i = 2 * (i2 + u * MOM_THREADS);
+        q2[i2 + u * MOM_THREADS] = double2{proj(x[u].x, i), proj(x[u].y, i + 1)};
+      }
+    }
+    for (; i2 < n2; i2 += MOM_THREADS) {
+      const double2 x = q2[i2];
+      q2[i2] = double2{proj(x.x, 2 * i2), proj(x.y, 2 * i2 + 1)};
+    }
+  } else {
+    for (int i = threadIdx.x; i < g.npts; i += MOM_THREADS) qc[i] = proj(qc[i], i);
   }
 }
 
@@ -282,15 +348,23 @@ int launch_df_combine(const double* adv, const double* q, double* out, double dt
 
 int launch_moments(const double* f, double* out, long long batch, int dim, int n, double L,
                    hipStream_t st) {
-  hipLaunchKernelGGL(k_moments, dim3((unsigned)batch), dim3(MOM_THREADS), 0, st, f, out,
-                     vgrid(dim, n), L);
+  if (dim == 2)
+    hipLaunchKernelGGL(k_moments<2>, dim3((unsigned)batch), dim3(MOM_THREADS), 0, st, f, out,
+                       vgrid(dim, n), L);
+  else
+    hipLaunchKernelGGL(k_moments<3>, dim3((unsigned)batch), dim3(MOM_THREADS), 0, st, f, out,
+                       vgrid(dim, n), L);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }
 
 int launch_project(double* q, long long batch, int dim, int n, double L, hipStream_t st) {
-  hipLaunchKernelGGL(k_project, dim3((unsigned)batch), dim3(MOM_THREADS), 0, st, q,
-                     vgrid(dim, n), L);
+  if (dim == 2)
+    hipLaunchKernelGGL(k_project<2>, dim3((unsigned)batch), dim3(MOM_THREADS), 0, st, q,
+                       vgrid(dim, n), L);
+  else
+    hipLaunchKernelGGL(k_project<3>, dim3((unsigned)batch), dim3(MOM_THREADS), 0, st, q,
+                       vgrid(dim, n), L);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }

@@ -178,3 +178,28 @@ def test_fast_sweep_solver_matches_oracle(pkg, need_gpu):
         assert abs(err - err_ref) <= 1e-12 * max(err_ref, 1e-300), (it, err, err_ref)
     err, steps = solver.solve(sol, n_iter=3)
     assert steps == 3 and err > 0
+
+
+@pytest.mark.parametrize("n,off", [(1, 0), (7, 0), (4096, 0), (4097, 0), (4097, 1), (1000001, 1)])
+def test_axpy_and_df_combine_bit_exact(pkg, need_gpu, n, off):
+    """The step updates f += a df (boltzmann.py:84) and df = adv + dt Q / kn (:160-164) equal
+    numpy bit for bit, on 16-byte aligned arrays (pair loads) and on arrays offset by one
+    element (single loads), with odd and even lengths."""
+    import torch
+    lib = pkg._lib.load()
+    st = torch.cuda.current_stream().cuda_stream
+    rng = np.random.default_rng(n + off)
+    y0, x, adv, q = (rng.standard_normal(n + off) for _ in range(4))
+    yd, xd, ad, qd = (torch.from_numpy(a).cuda() for a in (y0, x, adv, q))
+    out = torch.empty_like(qd)
+    p = lambda t: t.data_ptr() + 8 * off
+    assert lib.ib_axpy(p(yd), p(xd), 0.37, n, st) == 0
+    assert lib.ib_df_combine(p(ad), p(qd), p(out), 0.013, 0.0071, n, st) == 0
+    o2 = torch.empty_like(qd)
+    assert lib.ib_df_combine(None, p(qd), p(o2), 0.013, 0.0071, n, st) == 0
+    torch.cuda.synchronize()
+    ref_y = y0[off:] + 0.37 * x[off:]
+    assert np.array_equal(yd.cpu().numpy()[off:], ref_y)
+    c = (0.013 * q[off:]) / 0.0071
+    assert np.array_equal(out.cpu().numpy()[off:], adv[off:] + c)
+    assert np.array_equal(o2.cpu().numpy()[off:], c)
