@@ -134,6 +134,8 @@ class _Class:
             a.start = start
             start += a.cells
             act.append(a)
+        # new problems of one kind next to each other, so that df_from_q runs them in one launch
+        new = sorted(new, key=lambda af: (af[0].req.xlen > 0, af[0].cells, af[0].req.xlen, af[0].req.kn))
         if new:  # one host-to-device copy for all admitted problems
             parts.append(dev.to_device(np.concatenate(
                 [f0.reshape((a.cells,) + self.vshape) for a, f0 in new])))
@@ -161,38 +163,43 @@ class _Class:
             at.tune(self.coll, self.f, with_loss=True)
 
     def df_from_q(self, f, q, h):
-        """Stacked df of every problem: TVD transport (1D jobs) + h Q / kn."""
+        """Stacked df of every problem: TVD transport (1D jobs) + h Q / kn.
+
+        Adjacent problems of one kind run as one launch per kernel: homogeneous
+        problems with one kn, and 1D jobs with one (Nx, x length, kn), whose TVD
+        steps are independent slices of one (jobs, Nx, N^2) launch.
+        """
         t = dev.torch()
         lib = _lib.load()
         st = dev.stream_handle()
         out = t.empty_like(f)
-        run = None  # [first cell, cells, kn] of adjacent homogeneous problems with one kn
+        npc = f[0].numel() if f.shape[0] else 0
 
-        def combine(c0, nc, adv, kn):
+        def key(a):
+            r = a.req
+            return (r.cells, r.xlen, r.kn) if r.xlen > 0 else (0, 0.0, r.kn)
+
+        runs = []  # [first cell, cells, jobs, key]
+        for a in self.active:
+            k = key(a)
+            if runs and runs[-1][3] == k and runs[-1][0] + runs[-1][1] == a.start:
+                runs[-1][1] += a.cells
+                runs[-1][2] += 1
+            else:
+                runs.append([a.start, a.cells, 1, k])
+        for c0, nc, jobs, (nx, xlen, kn) in runs:
             sl = slice(c0, c0 + nc)
+            adv = None
+            if xlen > 0:
+                if nx + 2 < self.proto.n:
+                    # the reference slices s[LL-1:UL-1] along v_x and fails to broadcast
+                    raise ValueError("operands could not be broadcast together (nx + 2 < N)")
+                adv = t.empty_like(f[sl])
+                _lib.check(lib.ib_tvd_axis(f[sl].data_ptr(), adv.data_ptr(), jobs, nx, npc,
+                                           self.v1.data_ptr(), npc, float(h / (xlen / nx)), 2, 0,
+                                           1, 1, 0, st), "ib_tvd_axis")
             _lib.check(lib.ib_df_combine(dev.ptr(adv), q[sl].data_ptr(), out[sl].data_ptr(), float(h),
                                          float(kn), out[sl].numel(), st), "ib_df_combine")
-
-        for a in self.active:
-            if a.req.xlen > 0:
-                if run is not None:
-                    combine(run[0], run[1], None, run[2])
-                    run = None
-                sl = slice(a.start, a.start + a.cells)
-                adv = t.empty_like(f[sl])
-                dx = a.req.xlen / a.cells
-                _lib.check(lib.ib_tvd1d(f[sl].data_ptr(), adv.data_ptr(), a.cells, f[0].numel(),
-                                        self.v1.data_ptr(), float(h / dx), 2, 0, 1, 1,
-                                        self.proto.n, st), "ib_tvd1d")
-                combine(a.start, a.cells, adv, a.req.kn)
-            elif run is not None and run[2] == a.req.kn and run[0] + run[1] == a.start:
-                run[1] += a.cells
-            else:
-                if run is not None:
-                    combine(run[0], run[1], None, run[2])
-                run = [a.start, a.cells, a.req.kn]
-        if run is not None:
-            combine(run[0], run[1], None, run[2])
         return out
 
 
