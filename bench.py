@@ -392,6 +392,8 @@ SIDE_SHAPES = {
     # name: (dim, N, Nr, M, s, fused kernel, what)
     "2d32": (2, 32, 32, 8, S3, "k_fused2",
              "2D 32^2, Nr=32, M=8 (the homogeneous-cooling and 1D x 2D shape)"),
+    "2d64": (2, 64, 64, 8, 3.0, "k_fused2w",
+             "2D 64^2, Nr=64, M=8, s=3 (the 2D 64^2 shape of the uneven batch)"),
     "3d16": (3, 16, 16, 38, 3.0, "k_fusedc3",
              "3D 16^3, Nr=16, M=38 Lebedev, s=3 (the 3D shape of the uneven batch and serving mix)"),
 }
@@ -464,10 +466,12 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--check-cells", type=int, default=2)
     ap.add_argument("--cpu-cells", type=int, default=36)
-    ap.add_argument("--mode", choices=["bench", "profile", "profile2d", "profile3d16", "copy"],
+    ap.add_argument("--mode", choices=["bench", "profile", "profile2d", "profile2d64", "profile3d16",
+                                       "copy"],
                     default="bench",
                     help="profile: FP32 form only on --cells cells (for rocprofv3); "
                          "profile2d: the 2D 32^2 FP32 form only on --cells-2d cells; "
+                         "profile2d64: the 2D 64^2 FP32 form only on --cells-2d64 cells; "
                          "profile3d16: the 3D 16^3 FP32 form only on --cells-3d16 cells; "
                          "copy: a plain 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -483,6 +487,8 @@ def main():
                     help="cells of the 2D 32^2 side measurement (0 = skip)")
     ap.add_argument("--cells-3d16", type=int, default=2048,
                     help="cells of the 3D 16^3 side measurement (0 = skip)")
+    ap.add_argument("--cells-2d64", type=int, default=1024,
+                    help="cells of the 2D 64^2 side measurement (0 = skip)")
     ap.add_argument("--uneven-problems", type=int, default=256,
                     help="problems of the uneven homogeneous batch (0 = skip; 4096 = full size)")
     ap.add_argument("--no-hbm", action="store_true", help="skip the HBM-bound kernel measurements")
@@ -512,6 +518,10 @@ def main():
     if args.mode == "profile2d":
         print(json.dumps({"mode": "profile2d", **run_shape("2d32", args.cells_2d, args.steps, args.warmup,
                                                            False)}))
+        return
+    if args.mode == "profile2d64":
+        print(json.dumps({"mode": "profile2d64", **run_shape("2d64", args.cells_2d64, args.steps,
+                                                             args.warmup, False)}))
         return
     if args.mode == "profile3d16":
         print(json.dumps({"mode": "profile3d16", **run_shape("3d16", args.cells_3d16, args.steps,
@@ -623,6 +633,8 @@ def main():
         out["shape_2d_32"] = run_shape("2d32", args.cells_2d, args.steps, args.warmup, not args.no_fp64)
     if args.cells_3d16 > 0:
         out["shape_3d_16"] = run_shape("3d16", args.cells_3d16, args.steps, args.warmup, not args.no_fp64)
+    if args.cells_2d64 > 0:
+        out["shape_2d_64"] = run_shape("2d64", args.cells_2d64, args.steps, args.warmup, not args.no_fp64)
     if args.uneven_problems > 0:
         u32, r32 = uneven_run("fp32", args.uneven_problems, autotune=not args.no_uneven_autotune)
         out["uneven_batch"] = {"workload": "homogeneous cooling, 2D 16^2/32^2/64^2 + 3D 16^3/32^3, "

@@ -133,7 +133,7 @@ static int set_params(Plan* p, int n_param, const double* e_r, const double* gam
   IB_HIP(hipMalloc((void**)&p->d_lam64, (size_t)n_param * p->nq * sizeof(double)));
   rc = launch_tables(p, d_wr, st);
   if (rc) return rc;
-  if (p->dim == 2 && p->precision == IB_PREC_FP32 && (p->n == 16 || p->n == 32)) {
+  if (p->dim == 2 && p->precision == IB_PREC_FP32 && (p->n == 16 || p->n == 32 || p->n == 64)) {
     const size_t per = (size_t)p->nr * p->m * 2 * p->n * 2 * sizeof(float);
     if (!p->d_ephase) IB_HIP(hipMalloc(&p->d_ephase, per));
     if (n_param <= Plan::kMaxPhaseSets) IB_HIP(hipMalloc(&p->d_gphase, (size_t)n_param * per));

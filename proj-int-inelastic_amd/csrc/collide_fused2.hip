@@ -228,6 +228,15 @@ __global__ void k_f2_reduce(const cx<float>* __restrict__ part, cx<double>* __re
   acc64[gid] = cx<double>{re, im};
 }
 
+// Orders a wave's LDS accesses around an exchange between its own lanes: a wave's
+// LDS instructions execute in program order, so only the compiler must be kept
+// from moving them.
+__device__ __forceinline__ void wave_sync2() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 namespace c3 {
 constexpr int N = 16;
 constexpr int PAD = N + 1;
@@ -425,6 +434,247 @@ static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, con
   return IB_OK;
 }
 
+namespace c2w {
+constexpr int N = 64;
+constexpr int PAD = N + 1;
+constexpr int THREADS = 2 * N;  // two threads per pencil
+constexpr long long NPTS = (long long)N * N;
+}  // namespace c2w
+
+// One half of a 64-point transform (DIR = +1 inverse, -1 forward) of a pencil
+// a[0..63], computed by thread h of the pencil's two threads: on entry r[n] =
+// a[n] + (1 - 2h) a[n + 32] (the first radix-2 decimation-in-frequency stage);
+// r[n] is multiplied by tw[n] = 1 (h = 0) or exp(2 pi i n / 64) (h = 1, conjugated
+// for DIR = -1), read from LDS so that the twiddle constants take no registers,
+// and a 32-point FFT leaves X[2k + h] in r[k].
+template <int DIR>
+__device__ __forceinline__ void half_dif64(cx<float> (&r)[32], const cx<float>* tw) {
+#pragma unroll
+  for (int n = 1; n < 32; ++n) {
+    const cx<float> w = tw[n];
+    r[n] = DIR > 0 ? r[n] * w : r[n] * conj(w);
+  }
+  fft_reg<32, DIR>(r);
+}
+
+struct C2WArgs {
+  const cx<float>* fh;      // [cell][N][N], fhat / n
+  const float* f32;         // [cell][N][N]
+  const cx<float>* ephase;  // [pair][2][N] or null
+  const cx<float>* gphase;  // [param][pair][2][N] or null
+  cx<float>* part;          // [cell][nr][N][N]
+  const int* pidx;
+  long long batch;
+  const double* sigma;
+  const double* rj;
+  const double* wsig;
+  const double* crad;
+  const double* a;
+  const float* spg;
+  int nr, m, nq;
+  double L;
+};
+
+// 2D 64^2: one 128-thread workgroup per (cell, radial node j) loops over the M
+// angles with the 64 x 64 complex work array in LDS; each pass gives a pencil to
+// two adjacent lanes (half_dif64), so the sum over angles of a row's 64
+// frequencies takes 32 registers per thread.  Per pair: A rows (E * fhat from
+// L2, inverse along axis 1); B columns (inverse along axis 0, * f, forward along
+// axis 0; the column's two lanes are in one wave); C rows (forward along axis 1,
+// G * (.) and the sum).  Two workgroup barriers per pair.
+__global__ __launch_bounds__(c2w::THREADS, 2) void k_fused2w(C2WArgs A) {
+  using namespace c2w;
+  __shared__ cx<float> W[N][PAD];
+  __shared__ cx<float> tE[2][2][N];
+  __shared__ cx<float> tG[2][2][N];
+  __shared__ cx<float> twd[2][32];  // ones, then exp(2 pi i n / 64)
+  const int tid = threadIdx.x;
+  const long long id = blockIdx.x;
+  const int c8 = (int)(id % 8);  // the 8 * nr workgroups of 8 cells share one XCD per cell
+  const int j = (int)((id / 8) % A.nr);
+  const long long cell = (id / (8LL * A.nr)) * 8 + c8;
+  if (cell >= A.batch) return;  // uniform over the workgroup
+  const int ps = A.pidx ? A.pidx[cell] : 0;
+  if (tid < 64) {
+    const int n = tid % 32;
+    twd[tid / 32][n] = tid < 32 ? cx<float>{1.f, 0.f}
+                                : cx<float>{(float)TW64[n][0], (float)TW64[n][1]};
+  }
+  const int p = tid >> 1, h = tid & 1;  // pencil and half
+  const cx<float>* tw = twd[h];
+  const float sg = h ? -1.f : 1.f;
+  const double aj = A.a[ps * A.nr + j] * A.rj[j];
+  const double rpl = M_PI * A.rj[j] / A.L;
+  const int npairs = A.nr * A.m;
+  const cx<float>* fh = A.fh + cell * NPTS;
+  const float* fc = A.f32 + cell * NPTS;
+  // phase factors of pair (j, mm) along each axis: from the plan's tables when it
+  // has them, else FP64 sincos rounded to FP32 (inelastic_collisions.py:97-105)
+  auto load_tables = [&](int mm, int b) {
+    const int pair = j * A.m + mm;
+    for (int e = tid; e < 4 * N; e += THREADS) {
+      const int which = e / (2 * N), ax = (e / N) % 2, k = e % N;
+      cx<float> val;
+      if (which == 0 && A.ephase) {
+        val = A.ephase[((long long)pair * 2 + ax) * N + k];
+      } else if (which == 1 && A.gphase) {
+        val = A.gphase[(((long long)ps * npairs + pair) * 2 + ax) * N + k];
+      } else {
+        const double s = A.sigma[mm * 2 + ax];
+        const double kf = (double)kfreq(k, N);
+        const double th = which == 0 ? -(rpl * s * kf) : (aj * s * kf);
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        val = cx<float>{(float)cs, (float)sn};
+      }
+      (which ? tG : tE)[b][ax][k] = val;
+    }
+  };
+  cx<float> acc[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) acc[k] = cx<float>{0.f, 0.f};
+  load_tables(0, 0);
+  __syncthreads();
+  cx<float> r[32];
+  for (int mm = 0; mm < A.m; ++mm) {
+    const int b = mm & 1;
+    // A: row p = kx: E * fhat, inverse along axis 1 (ky -> y); the row factor
+    // E_x(kx) is constant along the row and applied to the outputs
+    {
+      const cx<float>* fr = fh + (long long)p * N;
+#pragma unroll
+      for (int n = 0; n < 32; ++n) {
+        const cx<float> a0 = tE[b][1][n] * fr[n];
+        const cx<float> a1 = tE[b][1][n + 32] * fr[n + 32];
+        r[n] = cx<float>{fmaf(sg, a1.re, a0.re), fmaf(sg, a1.im, a0.im)};
+        if ((n & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // 16 loads in flight at a time
+      }
+      half_dif64<+1>(r, tw);
+      const cx<float> ex = tE[b][0][p];
+#pragma unroll
+      for (int k = 0; k < 32; ++k) W[p][2 * k + h] = ex * r[k];
+    }
+    __syncthreads();
+    // B: column p = y: inverse along axis 0 (kx -> x), * f, forward along axis 0;
+    // the next pair's phase tables (their buffer's last readers, phases A and C of
+    // the previous pair, are behind the barrier above)
+    {
+#pragma unroll
+      for (int n = 0; n < 32; ++n) {
+        const cx<float> a0 = W[n][p], a1 = W[n + 32][p];
+        r[n] = cx<float>{fmaf(sg, a1.re, a0.re), fmaf(sg, a1.im, a0.im)};
+      }
+      half_dif64<+1>(r, tw);
+      // 32-bit offset made opaque per pair, so the 32 addresses are formed next to
+      // their loads instead of being held in registers across the pair loop
+      int toff = h * N + p;
+      asm volatile("" : "+v"(toff));
+#pragma unroll
+      for (int k = 0; k < 32; ++k) {
+        r[k] = fc[2 * k * N + toff] * r[k];
+        if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+      wave_sync2();  // both lanes of the column have read it
+#pragma unroll
+      for (int k = 0; k < 32; ++k) W[2 * k + h][p] = r[k];
+      wave_sync2();  // ... and written their halves
+#pragma unroll
+      for (int n = 0; n < 32; ++n) {
+        const cx<float> a0 = W[n][p], a1 = W[n + 32][p];
+        r[n] = cx<float>{fmaf(sg, a1.re, a0.re), fmaf(sg, a1.im, a0.im)};
+      }
+      half_dif64<-1>(r, tw);
+      wave_sync2();
+#pragma unroll
+      for (int k = 0; k < 32; ++k) W[2 * k + h][p] = r[k];
+    }
+    if (mm + 1 < A.m) load_tables(mm + 1, b ^ 1);
+    __syncthreads();
+    // C: row p = kx: forward along axis 1 (y -> ky), G * (.), sum over angles (the
+    // row's own two lanes are the only readers, so the next pair's phase A may
+    // overwrite it without a barrier)
+    {
+#pragma unroll
+      for (int n = 0; n < 32; ++n) {
+        const cx<float> a0 = W[p][n], a1 = W[p][n + 32];
+        r[n] = cx<float>{fmaf(sg, a1.re, a0.re), fmaf(sg, a1.im, a0.im)};
+      }
+      half_dif64<-1>(r, tw);
+      const cx<float> gx = (float)A.wsig[mm] * tG[b][0][p];
+#pragma unroll
+      for (int k = 0; k < 32; ++k) acc[k] = acc[k] + (gx * tG[b][1][2 * k + h]) * r[k];
+      wave_sync2();
+    }
+  }
+  // radial weight and sp(a_j r_j |k|), then this (cell, j)'s partial spectrum
+  const float cj = (float)A.crad[ps * A.nr + j];
+  const float* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
+  const int kx = kfreq(p, N);
+  cx<float>* out = A.part + ((long long)cell * A.nr + j) * NPTS + (long long)p * N;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    const int ky = kfreq(2 * k + h, N);
+    out[2 * k + h] = (cj * sp[kx * kx + ky * ky]) * acc[k];
+  }
+}
+
+static int run_fused2w(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                       int with_loss, double hb, hipStream_t st) {
+  using namespace c2w;
+  const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 1024);
+  const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
+  int rc;
+  if ((rc = P->fhat.ensure(sub * b64))) return rc;
+  if ((rc = P->acc.ensure(sub * b64))) return rc;
+  if ((rc = P->work.ensure(2 * sub * b64))) return rc;
+  const size_t part_b = (size_t)sub * P->nr * NPTS * sizeof(cx<float>);
+  const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<float>);
+  if ((rc = P->aux.ensure(part_b + fh_b + (size_t)sub * NPTS * sizeof(float)))) return rc;
+  cx<float>* part = (cx<float>*)P->aux.ptr;
+  cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + part_b);
+  float* f32 = (float*)((char*)fh32 + fh_b);
+  for (long long c0 = 0; c0 < batch; c0 += sub) {
+    const long long nb = std::min(sub, batch - c0);
+    const double* fc = f + c0 * NPTS;
+    const int* pc = pidx ? pidx + c0 : nullptr;
+    if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
+    const long long tot = nb * NPTS;
+    hipLaunchKernelGGL(k_f2_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       (const cx<double>*)P->fhat.ptr, fc, fh32, f32, tot, 1.0 / (double)NPTS);
+    IB_HIP(hipGetLastError());
+    C2WArgs A{};
+    A.fh = fh32;
+    A.f32 = f32;
+    A.ephase = (const cx<float>*)P->d_ephase;
+    A.gphase = (const cx<float>*)P->d_gphase;
+    A.part = part;
+    A.pidx = pc;
+    A.batch = nb;
+    A.sigma = P->d_sigma;
+    A.rj = P->d_rj;
+    A.wsig = P->d_wsig;
+    A.crad = P->d_crad;
+    A.a = P->d_a;
+    A.spg = (const float*)P->d_spg;
+    A.nr = P->nr;
+    A.m = P->m;
+    A.nq = P->nq;
+    A.L = P->L;
+    const long long groups = (nb + 7) / 8;
+    if ((rc = P->timed_begin(st))) return rc;
+    hipLaunchKernelGGL(k_fused2w, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
+    IB_HIP(hipGetLastError());
+    P->timed_end(st, nb);
+    hipLaunchKernelGGL(k_f2_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
+                       (cx<double>*)P->acc.ptr, nb, P->nr, NPTS);
+    IB_HIP(hipGetLastError());
+    if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
+                                   with_loss, hb, pc)))
+      return rc;
+  }
+  return IB_OK;
+}
+
 template <int N>
 static int run_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                       int with_loss, double hb, hipStream_t st) {
@@ -500,6 +750,7 @@ int collide_fused2(Plan* P, const double* f, double* q, long long batch, const i
   switch (P->n) {
     case 16: return run_fused2<16>(P, f, q, batch, pidx, with_loss, hb, st);
     case 32: return run_fused2<32>(P, f, q, batch, pidx, with_loss, hb, st);
+    case 64: return run_fused2w(P, f, q, batch, pidx, with_loss, hb, st);
     default:
       *handled = 0;
       return IB_OK;

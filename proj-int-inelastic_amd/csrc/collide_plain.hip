@@ -852,6 +852,7 @@ int staged_prologue_fp64(Plan* P, const double* f, long long nb, void* fhat64, v
   switch (P->dim * 100 + P->n) {
     case 216: return prologue_fp64<16, 2>(P, f, nb, fhat64, work);
     case 232: return prologue_fp64<32, 2>(P, f, nb, fhat64, work);
+    case 264: return prologue_fp64<64, 2>(P, f, nb, fhat64, work);
     case 316: return prologue_fp64<16, 3>(P, f, nb, fhat64, work);
     case 332: return prologue_fp64<32, 3>(P, f, nb, fhat64, work);
     default:
@@ -865,6 +866,7 @@ int staged_epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void
   switch (P->dim * 100 + P->n) {
     case 216: return epilogue_fp64<16, 2>(P, f, q, nb, acc64, fhat64, work, with_loss, hb, pidx);
     case 232: return epilogue_fp64<32, 2>(P, f, q, nb, acc64, fhat64, work, with_loss, hb, pidx);
+    case 264: return epilogue_fp64<64, 2>(P, f, q, nb, acc64, fhat64, work, with_loss, hb, pidx);
     case 316: return epilogue_fp64<16, 3>(P, f, q, nb, acc64, fhat64, work, with_loss, hb, pidx);
     case 332: return epilogue_fp64<32, 3>(P, f, q, nb, acc64, fhat64, work, with_loss, hb, pidx);
     default:

@@ -243,12 +243,13 @@ def test_fused_fp32_3d32_batch(pkg, need_gpu, with_loss, hb, prec):
     assert worst <= TOL32, worst
 
 
-@pytest.mark.parametrize("N,nr,M,s", [(16, 8, 8, 2.0), (32, 32, 8, 2.718491035932)])
+@pytest.mark.parametrize("N,nr,M,s", [(16, 8, 8, 2.0), (32, 32, 8, 2.718491035932), (64, 64, 8, 3.0)])
 @pytest.mark.parametrize("with_loss,hb", [(True, 0.0), (False, 0.05)])
 def test_fused_fp32_2d_batch(pkg, need_gpu, N, nr, M, s, with_loss, hb):
-    """Fused FP32 form of 2D grids (C2C, one N-thread group per quadrature pair):
-    13 cells with per-cell e and gamma, in sub-batches of 5 cells, against the
-    FP64 oracle cell by cell."""
+    """Fused FP32 forms of 2D grids (C2C; 16^2 and 32^2 one N-thread group per quadrature
+    pair, 64^2 one workgroup per (cell, radial node) with two threads per pencil): 13 cells
+    with per-cell e and gamma, in sub-batches of 5 cells, against the FP64 oracle cell by
+    cell."""
     import torch
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
     d = 2
@@ -296,12 +297,13 @@ def test_fused_fp32_3d16_batch(pkg, need_gpu, with_loss, hb):
     assert worst <= TOL32, worst
 
 
-def test_fused_fp32_2d_many_parameter_sets(pkg, need_gpu):
+@pytest.mark.parametrize("N,nr", [(16, 8), (64, 16)])
+def test_fused_fp32_2d_many_parameter_sets(pkg, need_gpu, N, nr):
     """More parameter sets than the precomputed phase tables hold (70 > 64): the 2D fused
-    kernel evaluates its gain phase factors itself; checked against the oracle."""
+    kernels evaluate their gain phase factors themselves; checked against the oracle."""
     import torch
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
-    d, N, nr, M, s = 2, 16, 8, 8, 2.0
+    d, M, s = 2, 8, 2.0
     cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 1.0)
     sm = SpectralMesh(cfg)
     coll = FSInelasticVHSCollision(cfg, sm, precision="fp32")
@@ -316,7 +318,7 @@ def test_fused_fp32_2d_many_parameter_sets(pkg, need_gpu):
 
 
 @pytest.mark.parametrize("d,N,nr,M,s", [(3, 32, 8, 14, 3.624654714576), (2, 32, 16, 8, 3.0),
-                                       (3, 16, 16, 38, 3.0)])
+                                       (3, 16, 16, 38, 3.0), (2, 64, 16, 8, 3.0)])
 def test_fused_fp32_batch_invariance_full_size(pkg, need_gpu, d, N, nr, M, s):
     """Size-independent property at batch sizes beyond the oracle's reach: 300 cells made of
     10 distinct cells repeated (several sub-batches, every XCD group position, per-cell e) give
