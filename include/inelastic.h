@@ -31,7 +31,7 @@ extern "C" {
 #define IB_PREC_FP64 0      /* plain FP64 form: C2C, staged batched FFTs        */
 #define IB_PREC_FP32 1      /* FP32 fast form: FP32 transforms and angular sums,
                                FP64 radial sums, final transform and loss term;
-                               fused kernels for 3D 32^3 and 16^3 and 2D 16^2 / 32^2 */
+                               fused kernels for 3D 32^3 and 16^3 and 2D 16^2 / 32^2 / 64^2 */
 #define IB_PREC_FP32_KAHAN 2 /* FP32 fast form whose angular sums in the fused
                                3D 32^3 kernel are Kahan-compensated (other
                                shapes: as IB_PREC_FP32) */
@@ -101,7 +101,7 @@ int ib_collide(ib_plan* plan, const double* f, double* q, long long batch,
  * one precision), each with its own cells f[i], q[i], batch[i], param_idx[i]
  * (param_idx may be NULL).  Every stage of the staged engine runs as ONE persistent
  * kernel launch over the work of all plans (up to 8 plans per launch); FP32 plans
- * whose shape has a fused kernel (3D 32^3 and 16^3, 2D 16^2 and 32^2) run it.  New: the reference evaluates
+ * whose shape has a fused kernel (3D 32^3 and 16^3, 2D 16^2, 32^2 and 64^2) run it.  New: the reference evaluates
  * one operator per call (src/collision/base.py:32-58). */
 int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, double* const* q,
                      const long long* batch, const int* const* param_idx, int with_loss,

@@ -11,8 +11,8 @@ reference's (Nr, M, n) complex tables.
 Extensions (keyword-only, all defaulting to the reference behaviour):
   precision   constructor keyword: "fp64" (plain FP64 form), "fp32" (FP32
               transforms and angular sums, FP64 radial sums, final transform
-              and loss term; fused kernels for 3D 32^3 and 16^3 and 2D 16^2
-              and 32^2) or "fp32k" (as "fp32" with Kahan-compensated angular
+              and loss term; fused kernels for 3D 32^3 and 16^3 and 2D 16^2,
+              32^2 and 64^2) or "fp32k" (as "fp32" with Kahan-compensated angular
               sums in the fused 3D 32^3 kernel).
   with_loss   constructor keyword: make ``__call__`` return gain - loss.
   e, e0       per-cell restitution arrays for ``collide`` (batched problems
