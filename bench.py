@@ -131,6 +131,20 @@ def cpu_baseline(nb_sample, seed):
     return out, ncpu, model
 
 
+def counters_from_profile():
+    """k_fused3 counter figures (VALU busy, LDS bandwidth, bank conflicts, occupancy) from the
+    committed SQ summary, or None."""
+    p = ROOT / "profiles" / "fused3_sq_summary.txt"
+    if not p.exists():
+        return None
+    out = {}
+    for line in p.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 2 and not line.startswith("#"):
+            out[parts[0]] = float(parts[1])
+    return out or None
+
+
 def traffic_from_profile():
     """HBM bytes per fused launch from the committed PMC summary, or None."""
     p = ROOT / "profiles" / "fused3_pmc_traffic.txt"
@@ -623,6 +637,7 @@ def main():
         out["fp64_plain_cell_evals_per_s"] = args.cells / res["fp64_s_per_step"]
         out["fp32_over_fp64"] = res["fp64_s_per_step"] / t32
         out["fp32_vs_fp64_gpu_rel_linf_max"] = res["fp32_vs_fp64_gpu_rel_linf_max"]
+    out["roofline"]["counters"] = counters_from_profile()
     pk = valu_peak()
     out["roofline"]["measured_valu_fma_peak_tflops"] = pk["fp32_tflops"]
     out["roofline"]["frac_of_measured_valu_peak"] = achieved / pk["fp32_tflops"]
