@@ -346,6 +346,94 @@ def hbm_kernels():
     return res
 
 
+def reference_configs(c2_steps=20):
+    """The reference's own two FP64 configurations through the solver step API, timed on the
+    device (eager and with each outer step replayed from a captured HIP graph):
+    C1 space-homogeneous 2D inelastic Maxwell molecules, 32^2 on [-6,6]^2, Nr = 32, M = 8,
+    e = 0.8, seeded two-Gaussian data (T = 0.5), forward Euler dt = 0.01, 100 steps (with
+    gain + loss, the conservative form of Q); C2 Nx = 200 x 32^2 on [-8,8]^2, hard spheres,
+    e = 0.9, eps = 1e-3, Sod-type Maxwellian data, minmod TVD, PFE with K = 2 inner steps of
+    eps and outer dt = 0.4 dx / 8 (the reference's gain-only __call__), c2_steps outer steps."""
+    import types
+    import torch
+    from proj_int_inelastic_amd.boltzmann import BoltzmannSolver0D, BoltzmannSolver1D
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    from proj_int_inelastic_amd.riemann.advection_1D import advection_1D
+    from proj_int_inelastic_amd.solution import Solution
+    from proj_int_inelastic_amd.solver import BC
+    out = {}
+    # C1
+    s1 = 2.718491035932
+    cfg = {"collision-model": {"dim": 2, "e": 0.8, "e0": 0.8, "gamma": 0.0},
+           "velocity-mesh": {"nv": 32, "s": s1, "nr": 32}, "circle-quad-rule": {"nphi": 8}}
+    sm = SpectralMesh(cfg)
+    dv = 2 * sm.L / 32
+    v = -sm.L + (np.arange(32) + 0.5) * dv
+    vx, vy = np.meshgrid(v, v, indexing="ij")
+    rng = np.random.default_rng(0)
+    cs = [np.array([sg, 0.0]) + rng.normal(0, 0.1, 2) for sg in (1.0, -1.0)]
+    f0 = sum(np.exp(-((vx - c[0]) ** 2 + (vy - c[1]) ** 2) / 1.0) for c in cs)
+    f0 /= f0.sum() * dv * dv
+    c1 = {"workload": "C1: 2D 32^2 homogeneous cooling, Maxwell molecules, e=0.8, Euler dt=0.01, 100 steps"}
+    for graph in (False, True):
+        coll = FSInelasticVHSCollision(cfg, sm, with_loss=True)
+        sol = Solution(types.SimpleNamespace(dim_x=0, dim_v=2, xmin=0.0, xmax=1.0, l=sm.L, nx=1, nv=32))
+        sol.state.f = f0.copy()
+        slv = BoltzmannSolver0D(kn=1.0, collision_operator=coll)
+        slv.dt = 0.01
+        slv.use_graph = graph
+        slv.evolve_to_time(sol, 0.05)  # warm-up: plans, scratch, graph capture
+        sol.state.f.copy_(torch.from_numpy(f0))  # in place: the captured graph stays valid
+        sol.t = 0.0
+        T0 = float(sol.state.T)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        slv.evolve_to_time(sol, 1.0)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        c1[("graph" if graph else "eager") + "_steps_per_s"] = 100 / sec
+        c1["T1_over_T0"] = float(sol.state.T) / T0
+    out["config1"] = c1
+    # C2
+    nx, n, s2 = 200, 32, S3
+    cfg = {"collision-model": {"dim": 2, "e": 0.9, "e0": 0.9, "gamma": 1.0},
+           "velocity-mesh": {"nv": n, "s": s2, "nr": 32}, "circle-quad-rule": {"nphi": 8}}
+    sm = SpectralMesh(cfg)
+    v = sm.v_center
+    vx, vy = np.meshgrid(v, v, indexing="ij")
+    x = (np.arange(nx) + 0.5) / nx
+    rho = np.where(x < 0.5, 1.0, 0.125)[:, None, None]
+    T = np.where(x < 0.5, 1.0, 0.8)[:, None, None]
+    f0 = rho / (2 * math.pi * T) * np.exp(-(vx ** 2 + vy ** 2) / (2 * T))
+    dt = 0.4 * (1.0 / nx) / 8
+    c2 = {"workload": "C2: Nx=200 x 2D 32^2 Sod-type Riemann data, hard spheres, e=0.9, eps=1e-3, "
+                      "minmod TVD, PFE K=2 inner 1e-3, outer 0.4 dx/8, %d outer steps" % c2_steps}
+    for graph in (False, True):
+        coll = FSInelasticVHSCollision(cfg, sm)
+        sol = Solution(types.SimpleNamespace(dim_x=1, dim_v=2, xmin=0.0, xmax=1.0, l=sm.L, nx=nx, nv=n))
+        slv = BoltzmannSolver1D(kn=1e-3, riemann_solver=advection_1D, collision_operator=coll)
+        slv.bc_lower[0] = BC.extrap
+        slv.bc_upper[0] = BC.extrap
+        slv.time_integrator = "PFE"
+        slv.inner_steps = [2]
+        slv.inner_dt = [1e-3]
+        slv.dt = dt
+        slv.use_graph = graph
+        sol.state.f = f0.copy()
+        slv.evolve_to_time(sol, 2 * dt)  # warm-up
+        sol.state.f.copy_(torch.from_numpy(f0))
+        sol.t = 0.0
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        slv.evolve_to_time(sol, c2_steps * dt)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        c2[("graph" if graph else "eager") + "_steps_per_s"] = c2_steps / sec
+        c2[("graph" if graph else "eager") + "_cell_evals_per_s"] = c2_steps * 2 * nx / sec
+    out["config2"] = c2
+    return out
+
+
 def uneven_requests(nprob, precision, seed=9):
     """The uneven batch of homogeneous cooling problems.
 
@@ -506,6 +594,8 @@ def main():
     ap.add_argument("--uneven-problems", type=int, default=256,
                     help="problems of the uneven homogeneous batch (0 = skip; 4096 = full size)")
     ap.add_argument("--no-hbm", action="store_true", help="skip the HBM-bound kernel measurements")
+    ap.add_argument("--no-ref-configs", action="store_true",
+                    help="skip the step-loop timing of the reference's 2D configurations")
     ap.add_argument("--no-uneven-autotune", action="store_true",
                     help="run the FP32 uneven batch with the default launch shapes")
     args = ap.parse_args()
@@ -650,6 +740,8 @@ def main():
         out["shape_3d_16"] = run_shape("3d16", args.cells_3d16, args.steps, args.warmup, not args.no_fp64)
     if args.cells_2d64 > 0:
         out["shape_2d_64"] = run_shape("2d64", args.cells_2d64, args.steps, args.warmup, not args.no_fp64)
+    if not args.no_ref_configs:
+        out["reference_configs_fp64"] = reference_configs()
     if args.uneven_problems > 0:
         u32, r32 = uneven_run("fp32", args.uneven_problems, autotune=not args.no_uneven_autotune)
         out["uneven_batch"] = {"workload": "homogeneous cooling, 2D 16^2/32^2/64^2 + 3D 16^3/32^3, "

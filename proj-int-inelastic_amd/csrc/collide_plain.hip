@@ -41,8 +41,9 @@ struct AArgs {
   const cx<T>* src;
   const cx<T>* src2;
   cx<T>* dst;
+  cx<T>* part;  // A_OUT_PAIR with S > 1: per-(cell, split) partial sums
   long long items;
-  int P, p0, first;
+  int P, p0, first, S;
   const int* pidx;
   long long cell0;
   const double* sigma;
@@ -133,13 +134,22 @@ __device__ __forceinline__ void axis0_body(const AArgs<T>& A, int bx, long long 
 #pragma unroll
     for (int i = 0; i < N; ++i) d[(long long)i * NP] = v[i];
   } else if (MODE == A_OUT_PAIR) {
-    const long long cell = item;
+    // S > 1 splits the chunk's pairs of a cell over S items (small batches, whose
+    // cells alone would leave most CUs idle); split s sums pairs [lo, hi) into its
+    // own partial, and k_split_add sums the partials in split order.
+    const int S = A.S > 1 ? A.S : 1;
+    const long long cell = item / S;
+    const int split = (int)(item % S);
+    const int lo = (int)(((long long)split * A.P) / S), hi = (int)(((long long)(split + 1) * A.P) / S);
+    const int tmax = (A.P + S - 1) / S;  // the same trip count in every item of the workgroup
     const int ps = (active && A.pidx) ? A.pidx[A.cell0 + cell] : 0;
-    for (int pl = 0; pl < A.P; ++pl) {
-      const int pair = A.p0 + pl;
+    for (int t = 0; t < tmax; ++t) {
+      const int pl = lo + t;
+      const bool on = active && pl < hi;
+      const int pair = A.p0 + (on ? pl : lo);
       const int j = pair / A.m, mm = pair % A.m;
       __syncthreads();
-      if (active) {
+      if (on) {
         const double aj = A.a[ps * A.nr + j] * A.rj[j];
         for (int t = lane; t < D * N; t += NPW) {
           const int ax = t / N, k = t % N;
@@ -150,7 +160,7 @@ __device__ __forceinline__ void axis0_body(const AArgs<T>& A, int bx, long long 
         }
       }
       __syncthreads();
-      if (!active) continue;
+      if (!on) continue;
       cx<T> gr = tab[sub][1][k1];
       if (D == 3) gr = gr * tab[sub][2][k2];
       const T wj = (T)(A.crad[ps * A.nr + j] * A.wsig[mm]);
@@ -163,16 +173,22 @@ __device__ __forceinline__ void axis0_body(const AArgs<T>& A, int bx, long long 
       for (int i = 0; i < N; ++i) {
         const int ki = kfreq(i, N);
         const T g = wj * sp[ki * ki + qrest];
-        const cx<T> t = (g * (tab[sub][0][i] * gr)) * v[i];
-        accs[i][tid] = (pl == 0) ? t : accs[i][tid] + t;
+        const cx<T> tv = (g * (tab[sub][0][i] * gr)) * v[i];
+        accs[i][tid] = (t == 0) ? tv : accs[i][tid] + tv;
       }
     }
     if (!active) return;
+    if (S > 1) {
+      cx<T>* d = A.part + item * NPTS + p;
+#pragma unroll
+      for (int i = 0; i < N; ++i) d[(long long)i * NP] = accs[i][tid];
+      return;
+    }
     cx<T>* d = A.dst + cell * NPTS + p;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      const cx<T> t = accs[i][tid];
-      d[(long long)i * NP] = A.first ? t : d[(long long)i * NP] + t;
+      const cx<T> tv = accs[i][tid];
+      d[(long long)i * NP] = A.first ? tv : d[(long long)i * NP] + tv;
     }
   } else if (MODE == A_OUT_PRO) {
     if (!active) return;
@@ -419,6 +435,18 @@ static int launch_a(const AArgs<T>& A, hipStream_t st) {
   return IB_OK;
 }
 
+// acc[c] (+)= sum over s of part[c][s], in split order.
+template <typename T>
+__global__ void k_split_add(const cx<T>* __restrict__ part, cx<T>* __restrict__ acc, long long nb,
+                            int S, long long npts, int first) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= nb * npts) return;
+  const long long c = gid / npts, e = gid % npts;
+  cx<T> sum = part[c * S * npts + e];
+  for (int s = 1; s < S; ++s) sum = sum + part[(c * S + s) * npts + e];
+  acc[gid] = first ? sum : acc[gid] + sum;
+}
+
 template <typename T, int N, int D, int MODE>
 static int launch_b(const BArgs<T>& B, hipStream_t st) {
   if constexpr (D == 2) {
@@ -514,9 +542,29 @@ static int collide_staged(Plan* P, const double* f, double* q, long long batch, 
       P->timed_end(st, nb * np);
       A.src = wk;
       A.dst = ac;
-      A.items = nb;
       A.first = (p0 == 0);
-      IB_TRY((launch_a<T, N, D, A_OUT_PAIR>(A, st)));
+      // split the pair sum when the cells alone give fewer than 512 workgroups
+      const long long wg0 = (long long)G::BLK * ((nb + G::SUBS - 1) / G::SUBS);
+      long long S = 1;
+      if (wg0 < 512 && np >= 4) {
+        S = std::min<long long>((1024 + wg0 - 1) / wg0, np / 2);
+        S = std::min<long long>(S, std::max<long long>(1, (512LL << 20) / ((long long)nb * arr)));
+      }
+      if (S > 1) {
+        IB_TRY(P->aux.ensure((size_t)(nb * S) * arr));
+        A.part = (cx<T>*)P->aux.ptr;
+        A.S = (int)S;
+        A.items = nb * S;
+        IB_TRY((launch_a<T, N, D, A_OUT_PAIR>(A, st)));
+        const long long tot = nb * NPTS;
+        hipLaunchKernelGGL(k_split_add<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                           (const cx<T>*)A.part, ac, nb, (int)S, NPTS, (int)A.first);
+        IB_HIP(hipGetLastError());
+        A.S = 1;
+      } else {
+        A.items = nb;
+        IB_TRY((launch_a<T, N, D, A_OUT_PAIR>(A, st)));
+      }
     }
     // epilogue: q = Re ifft(acc + hb lap fhat) - f Re ifft(Lambda fhat)
     A.src = ac;
@@ -747,6 +795,7 @@ static int collide_group_t(Plan** P, int np, const double* const* f, double* con
   // quadrature pairs: round r serves chunk r of every plan that has one
   for (int r = 0;; ++r) {
     GSegs<T> Gi{}, Gb{}, Go{};
+    int owner[kMaxSeg];  // segment of this round -> index into segs
     int live = 0;
     for (int k = 0; k < ns; ++k) {
       const long long p0 = (long long)r * segs[k].pc;
@@ -768,13 +817,48 @@ static int collide_group_t(Plan** P, int np, const double* const* f, double* con
       g.A.items = segs[k].nb;
       g.A.first = (r == 0);
       Go.s[live] = g;
+      owner[live] = k;
       ++live;
     }
     if (live == 0) break;
     Gi.nseg = Gb.nseg = Go.nseg = live;
     IB_TRY((launch_group<T, A_IN_PAIR, true>(Gi, st)));
     IB_TRY((launch_group<T, B_PAIR, false>(Gb, st)));
+    // split the pair sums (as in collide_staged) when all segments' cells together give
+    // fewer than 512 workgroups
+    long long wg0 = 0;
+    for (int k = 0; k < live; ++k) {
+      long long nbk;
+      size_t l;
+      seg_shape<T, A_OUT_PAIR, true>(Go.s[k].kind, Go.s[k].A.items, &nbk, &l);
+      wg0 += nbk;
+    }
+    int split[kMaxSeg];
+    for (int k = 0; k < live; ++k) {
+      split[k] = 1;
+      if (wg0 >= 512) continue;
+      Plan* pk = segs[owner[k]].p;
+      const long long nbk = Go.s[k].A.items;
+      const long long arr = pk->npts * (long long)sizeof(cx<T>);
+      long long S = std::min<long long>((1024 + wg0 - 1) / wg0, Go.s[k].A.P / 2);
+      S = std::min<long long>(S, std::max<long long>(1, (512LL << 20) / (nbk * arr)));
+      if (S < 2) continue;
+      IB_TRY(pk->aux.ensure((size_t)(nbk * S * arr)));
+      Go.s[k].A.part = (cx<T>*)pk->aux.ptr;
+      Go.s[k].A.S = (int)S;
+      Go.s[k].A.items = nbk * S;
+      split[k] = (int)S;
+    }
     IB_TRY((launch_group<T, A_OUT_PAIR, true>(Go, st)));
+    for (int k = 0; k < live; ++k) {
+      if (split[k] == 1) continue;
+      const long long nbk = Go.s[k].A.items / split[k], pts = segs[owner[k]].p->npts;
+      const long long tot = nbk * pts;
+      hipLaunchKernelGGL(k_split_add<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                         (const cx<T>*)Go.s[k].A.part, Go.s[k].A.dst, nbk, split[k], pts,
+                         Go.s[k].A.first);
+      IB_HIP(hipGetLastError());
+    }
   }
   // epilogue
   for (int k = 0; k < ns; ++k) {
