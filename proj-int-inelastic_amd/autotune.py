@@ -2,8 +2,8 @@
 
 For a plan and a batch size it times the candidate settings of the engine the
 plan uses and keeps the fastest:
-  * fused FP32 forms (3D 32^3; 2D 16^2, 32^2, 64^2 and 3D 16^3): cells per fused
-    launch (3D 32^3 64 .. 512, the others 256 .. 4096);
+  * fused forms (FP32 3D 32^3; FP32 2D 16^2, 32^2, 64^2 and 3D 16^3; FP64 2D
+    32^2): cells per fused launch (3D 32^3 64 .. 512, the others 256 .. 4096);
   * staged form: the scratch budget, which sets the quadrature pairs per chunk
     (items per pass launch) — 256 MiB .. 4 GiB.
 Results are cached in memory and in a JSON file (``INELASTIC_TUNE_CACHE`` or
@@ -58,6 +58,8 @@ def candidates(coll, batch):
         return [{"fused_cells": c} for c in (64, 128, 256, 512) if c <= max(64, batch)]
     if coll.precision in ("fp32", "fp32k") and ((coll.ndim == 2 and coll.sm.nv in (16, 32, 64)) or
                                                 (coll.ndim == 3 and coll.sm.nv == 16)):
+        return [{"fused_cells": c} for c in (256, 1024, 4096) if c <= max(256, batch)]
+    if coll.precision == "fp64" and coll.ndim == 2 and coll.sm.nv == 32:
         return [{"fused_cells": c} for c in (256, 1024, 4096) if c <= max(256, batch)]
     return [{"scratch_budget": b << 20} for b in (256, 1024, 4096)]
 

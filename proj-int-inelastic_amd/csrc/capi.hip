@@ -133,8 +133,10 @@ static int set_params(Plan* p, int n_param, const double* e_r, const double* gam
   IB_HIP(hipMalloc((void**)&p->d_lam64, (size_t)n_param * p->nq * sizeof(double)));
   rc = launch_tables(p, d_wr, st);
   if (rc) return rc;
-  if (p->dim == 2 && p->precision == IB_PREC_FP32 && (p->n == 16 || p->n == 32 || p->n == 64)) {
-    const size_t per = (size_t)p->nr * p->m * 2 * p->n * 2 * sizeof(float);
+  const bool fp32_2d = p->precision == IB_PREC_FP32 && (p->n == 16 || p->n == 32 || p->n == 64);
+  const bool fp64_2d = p->precision == IB_PREC_FP64 && p->n == 32;
+  if (p->dim == 2 && (fp32_2d || fp64_2d)) {  // the 2D fused kernels' phase tables
+    const size_t per = (size_t)p->nr * p->m * 2 * p->n * 2 * (fp64_2d ? sizeof(double) : sizeof(float));
     if (!p->d_ephase) IB_HIP(hipMalloc(&p->d_ephase, per));
     if (n_param <= Plan::kMaxPhaseSets) IB_HIP(hipMalloc(&p->d_gphase, (size_t)n_param * per));
     rc = launch_phase_tables2(p, st);
@@ -309,11 +311,9 @@ int ib_collide(ib_plan* p, const double* f, double* q, long long batch, const in
   if (batch == 0) return IB_OK;
   hipStream_t st = (hipStream_t)stream;
   p->stream = st;
-  if (p->precision == IB_PREC_FP32) {
-    int handled = 0;
-    int rc = collide_fused(p, f, q, batch, param_idx, with_loss, heat_bath, st, &handled);
-    if (rc || handled) return rc;
-  }
+  int handled = 0;  // shapes with a fused kernel run it
+  int rc = collide_fused(p, f, q, batch, param_idx, with_loss, heat_bath, st, &handled);
+  if (rc || handled) return rc;
   return collide_plain(p, f, q, batch, param_idx, with_loss, heat_bath, 0, st);
 }
 
@@ -344,7 +344,7 @@ int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, 
     if (batch[i] == 0) continue;
     p->stream = st;
     const int* pi = param_idx ? param_idx[i] : nullptr;
-    if (prec == IB_PREC_FP32) {  // shapes with a fused FP32 kernel run it
+    {  // shapes with a fused kernel run it
       int handled = 0;
       int rc = collide_fused(p, f[i], q[i], batch[i], pi, with_loss, heat_bath, st, &handled);
       if (rc) return rc;

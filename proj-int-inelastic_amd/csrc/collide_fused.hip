@@ -39,7 +39,7 @@ namespace ib {
 int staged_prologue_fp64(Plan* P, const double* f, long long nb, void* fhat64, void* work);
 int staged_epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void* acc64,
                          void* fhat64, void* work, int with_loss, double hb, const int* pidx);
-// fused C2C FP32 engines: 2D and 3D 16^3 (collide_fused2.hip)
+// fused C2C engines: FP32 2D and 3D 16^3, FP64 2D 32^2 (collide_fused2.hip)
 int collide_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                    int with_loss, double hb, hipStream_t st, int* handled);
 
@@ -446,7 +446,8 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
                   int with_loss, double hb, hipStream_t st, int* handled) {
   using namespace f3;
   *handled = 0;
-  if (P->precision == IB_PREC_FP32 && (P->dim == 2 || P->n == 16))
+  if ((P->precision == IB_PREC_FP32 && (P->dim == 2 || P->n == 16)) ||
+      (P->precision == IB_PREC_FP64 && P->dim == 2 && P->n == 32))
     return collide_fused2(P, f, q, batch, pidx, with_loss, hb, st, handled);
   if (!(P->dim == 3 && P->n == 32 && P->precision == IB_PREC_FP32)) return IB_OK;
   *handled = 1;

@@ -170,10 +170,11 @@ __global__ __launch_bounds__(F2Shape<N>::THREADS) void k_fused2(F2Args A) {
 
 // Phase tables of every quadrature pair (j, m), axis a and frequency index k:
 // E = exp(-i pi r_j sigma_a kf / L) and, per parameter set, G = exp(i a_j r_j sigma_a kf)
-// (the separable factors of inelastic_collisions.py:97-105), FP64 sincos rounded to FP32.
-__global__ void k_phase_tables2(cx<float>* ephase, cx<float>* gphase, const double* rj,
-                                const double* sigma, const double* a, int nr, int m, int n,
-                                int n_param, double L) {
+// (the separable factors of inelastic_collisions.py:97-105), FP64 sincos (rounded to
+// FP32 for FP32 plans).
+template <typename T>
+__global__ void k_phase_tables2(cx<T>* ephase, cx<T>* gphase, const double* rj, const double* sigma,
+                                const double* a, int nr, int m, int n, int n_param, double L) {
   const long long per = (long long)nr * m * 2 * n;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long total = per * (1 + (gphase ? n_param : 0));
@@ -186,42 +187,52 @@ __global__ void k_phase_tables2(cx<float>* ephase, cx<float>* gphase, const doub
   double sn, cs;
   if (gid < per) {
     sincos(-(M_PI * rj[j] * s * kf) / L, &sn, &cs);
-    ephase[t] = cx<float>{(float)cs, (float)sn};
+    ephase[t] = cx<T>{(T)cs, (T)sn};
   } else {
     const int ps = (int)(gid / per - 1);
     sincos(a[ps * nr + j] * rj[j] * s * kf, &sn, &cs);
-    gphase[gid - per] = cx<float>{(float)cs, (float)sn};
+    gphase[gid - per] = cx<T>{(T)cs, (T)sn};
   }
 }
 
 int launch_phase_tables2(Plan* P, hipStream_t st) {
   const long long per = (long long)P->nr * P->m * 2 * P->n;
   const long long total = per * (1 + (P->d_gphase ? P->n_param : 0));
-  hipLaunchKernelGGL(k_phase_tables2, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
-                     (cx<float>*)P->d_ephase, (cx<float>*)P->d_gphase, P->d_rj, P->d_sigma, P->d_a,
-                     P->nr, P->m, P->n, P->n_param, P->L);
+  const dim3 grid((unsigned)((total + 255) / 256));
+  if (P->precision == IB_PREC_FP64)
+    hipLaunchKernelGGL(k_phase_tables2<double>, grid, dim3(256), 0, st, (cx<double>*)P->d_ephase,
+                       (cx<double>*)P->d_gphase, P->d_rj, P->d_sigma, P->d_a, P->nr, P->m, P->n,
+                       P->n_param, P->L);
+  else
+    hipLaunchKernelGGL(k_phase_tables2<float>, grid, dim3(256), 0, st, (cx<float>*)P->d_ephase,
+                       (cx<float>*)P->d_gphase, P->d_rj, P->d_sigma, P->d_a, P->nr, P->m, P->n,
+                       P->n_param, P->L);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }
 
+// fhat / n and f in the fused kernels' type.
+template <typename T>
 __global__ void k_f2_pack(const cx<double>* __restrict__ fhat, const double* __restrict__ f,
-                          cx<float>* __restrict__ fh32, float* __restrict__ f32, long long total,
+                          cx<T>* __restrict__ fh32, T* __restrict__ f32, long long total,
                           double inv_n) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= total) return;
-  f32[gid] = (float)f[gid];
+  f32[gid] = (T)f[gid];
   const cx<double> v = fhat[gid];
-  fh32[gid] = cx<float>{(float)(v.re * inv_n), (float)(v.im * inv_n)};
+  fh32[gid] = cx<T>{(T)(v.re * inv_n), (T)(v.im * inv_n)};
 }
 
-__global__ void k_f2_reduce(const cx<float>* __restrict__ part, cx<double>* __restrict__ acc64,
+// FP64 sum of each cell's wpc partial spectra, in partial order.
+template <typename T>
+__global__ void k_f2_reduce(const cx<T>* __restrict__ part, cx<double>* __restrict__ acc64,
                             long long nb, int wpc, long long npts) {
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= nb * npts) return;
   const long long c = gid / npts, e = gid % npts;
   double re = 0.0, im = 0.0;
   for (int w = 0; w < wpc; ++w) {
-    const cx<float> x = part[(c * wpc + w) * npts + e];
+    const cx<T> x = part[(c * wpc + w) * npts + e];
     re += (double)x.re;
     im += (double)x.im;
   }
@@ -400,7 +411,7 @@ static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, con
     const int* pc = pidx ? pidx + c0 : nullptr;
     if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
     const long long tot = nb * NPTS;
-    hipLaunchKernelGGL(k_f2_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_f2_pack<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                        (const cx<double>*)P->fhat.ptr, fc, fh32, f32, tot, 1.0 / (double)NPTS);
     IB_HIP(hipGetLastError());
     C3Args A{};
@@ -424,7 +435,7 @@ static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, con
     hipLaunchKernelGGL(k_fusedc3, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
-    hipLaunchKernelGGL(k_f2_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
+    hipLaunchKernelGGL(k_f2_reduce<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
                        (cx<double>*)P->acc.ptr, nb, P->nr, NPTS);
     IB_HIP(hipGetLastError());
     if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
@@ -434,35 +445,29 @@ static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, con
   return IB_OK;
 }
 
-namespace c2w {
-constexpr int N = 64;
-constexpr int PAD = N + 1;
-constexpr int THREADS = 2 * N;  // two threads per pencil
-constexpr long long NPTS = (long long)N * N;
-}  // namespace c2w
-
-// One half of a 64-point transform (DIR = +1 inverse, -1 forward) of a pencil
-// a[0..63], computed by thread h of the pencil's two threads: on entry r[n] =
-// a[n] + (1 - 2h) a[n + 32] (the first radix-2 decimation-in-frequency stage);
-// r[n] is multiplied by tw[n] = 1 (h = 0) or exp(2 pi i n / 64) (h = 1, conjugated
+// One half of an N-point transform (DIR = +1 inverse, -1 forward) of a pencil
+// a[0..N-1], computed by lane h of the pencil's two lanes: on entry r[n] =
+// a[n] + (1 - 2h) a[n + N/2] (the first radix-2 decimation-in-frequency stage);
+// r[n] is multiplied by tw[n] = 1 (h = 0) or exp(2 pi i n / N) (h = 1, conjugated
 // for DIR = -1), read from LDS so that the twiddle constants take no registers,
-// and a 32-point FFT leaves X[2k + h] in r[k].
-template <int DIR>
-__device__ __forceinline__ void half_dif64(cx<float> (&r)[32], const cx<float>* tw) {
+// and an N/2-point FFT leaves X[2k + h] in r[k].
+template <int DIR, int H, typename T>
+__device__ __forceinline__ void half_dif(cx<T> (&r)[H], const cx<T>* tw) {
 #pragma unroll
-  for (int n = 1; n < 32; ++n) {
-    const cx<float> w = tw[n];
+  for (int n = 1; n < H; ++n) {
+    const cx<T> w = tw[n];
     r[n] = DIR > 0 ? r[n] * w : r[n] * conj(w);
   }
-  fft_reg<32, DIR>(r);
+  fft_reg<H, DIR>(r);
 }
 
-struct C2WArgs {
-  const cx<float>* fh;      // [cell][N][N], fhat / n
-  const float* f32;         // [cell][N][N]
-  const cx<float>* ephase;  // [pair][2][N] or null
-  const cx<float>* gphase;  // [param][pair][2][N] or null
-  cx<float>* part;          // [cell][nr][N][N]
+template <typename T>
+struct CWArgs {
+  const cx<T>* fh;      // [cell][N][N], fhat / n
+  const T* f;           // [cell][N][N]
+  const cx<T>* ephase;  // [pair][2][N] or null
+  const cx<T>* gphase;  // [param][pair][2][N] or null
+  cx<T>* part;          // [cell][nr][N][N]
   const int* pidx;
   long long batch;
   const double* sigma;
@@ -470,24 +475,27 @@ struct C2WArgs {
   const double* wsig;
   const double* crad;
   const double* a;
-  const float* spg;
+  const T* spg;
   int nr, m, nq;
   double L;
 };
 
-// 2D 64^2: one 128-thread workgroup per (cell, radial node j) loops over the M
-// angles with the 64 x 64 complex work array in LDS; each pass gives a pencil to
-// two adjacent lanes (half_dif64), so the sum over angles of a row's 64
-// frequencies takes 32 registers per thread.  Per pair: A rows (E * fhat from
-// L2, inverse along axis 1); B columns (inverse along axis 0, * f, forward along
-// axis 0; the column's two lanes are in one wave); C rows (forward along axis 1,
-// G * (.) and the sum).  Two workgroup barriers per pair.
-__global__ __launch_bounds__(c2w::THREADS, 2) void k_fused2w(C2WArgs A) {
-  using namespace c2w;
-  __shared__ cx<float> W[N][PAD];
-  __shared__ cx<float> tE[2][2][N];
-  __shared__ cx<float> tG[2][2][N];
-  __shared__ cx<float> twd[2][32];  // ones, then exp(2 pi i n / 64)
+// 2D grids whose pencil would not fit in one lane's registers next to the sum over
+// angles (FP32 64^2, FP64 32^2): one 2N-thread workgroup per (cell, radial node j)
+// loops over the M angles with the N x N complex work array in LDS; each pass
+// gives a pencil to two adjacent lanes (half_dif), so the sum over angles of a
+// row's N frequencies takes N/2 complex registers per lane.  Per pair: A rows
+// (E * fhat from L2, inverse along axis 1); B columns (inverse along axis 0, * f,
+// forward along axis 0; the column's two lanes are in one wave); C rows (forward
+// along axis 1, G * (.) and the sum).  Two workgroup barriers per pair.
+template <typename T, int N>
+__global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
+  constexpr int H = N / 2, THREADS = 2 * N, PAD = N + 1;
+  constexpr long long NPTS = (long long)N * N;
+  __shared__ cx<T> W[N][PAD];
+  __shared__ cx<T> tE[2][2][N];
+  __shared__ cx<T> tG[2][2][N];
+  __shared__ cx<T> twd[2][H];  // ones, then exp(2 pi i n / N)
   const int tid = threadIdx.x;
   const long long id = blockIdx.x;
   const int c8 = (int)(id % 8);  // the 8 * nr workgroups of 8 cells share one XCD per cell
@@ -495,26 +503,26 @@ __global__ __launch_bounds__(c2w::THREADS, 2) void k_fused2w(C2WArgs A) {
   const long long cell = (id / (8LL * A.nr)) * 8 + c8;
   if (cell >= A.batch) return;  // uniform over the workgroup
   const int ps = A.pidx ? A.pidx[cell] : 0;
-  if (tid < 64) {
-    const int n = tid % 32;
-    twd[tid / 32][n] = tid < 32 ? cx<float>{1.f, 0.f}
-                                : cx<float>{(float)TW64[n][0], (float)TW64[n][1]};
+  if (tid < N) {
+    const int n = tid % H;
+    twd[tid / H][n] = tid < H ? cx<T>{(T)1, (T)0}
+                              : cx<T>{(T)TW64[n * (64 / N)][0], (T)TW64[n * (64 / N)][1]};
   }
   const int p = tid >> 1, h = tid & 1;  // pencil and half
-  const cx<float>* tw = twd[h];
-  const float sg = h ? -1.f : 1.f;
+  const cx<T>* tw = twd[h];
+  const T sg = h ? (T)-1 : (T)1;
   const double aj = A.a[ps * A.nr + j] * A.rj[j];
   const double rpl = M_PI * A.rj[j] / A.L;
   const int npairs = A.nr * A.m;
-  const cx<float>* fh = A.fh + cell * NPTS;
-  const float* fc = A.f32 + cell * NPTS;
+  const cx<T>* fh = A.fh + cell * NPTS;
+  const T* fc = A.f + cell * NPTS;
   // phase factors of pair (j, mm) along each axis: from the plan's tables when it
-  // has them, else FP64 sincos rounded to FP32 (inelastic_collisions.py:97-105)
+  // has them, else FP64 sincos (inelastic_collisions.py:97-105)
   auto load_tables = [&](int mm, int b) {
     const int pair = j * A.m + mm;
     for (int e = tid; e < 4 * N; e += THREADS) {
       const int which = e / (2 * N), ax = (e / N) % 2, k = e % N;
-      cx<float> val;
+      cx<T> val;
       if (which == 0 && A.ephase) {
         val = A.ephase[((long long)pair * 2 + ax) * N + k];
       } else if (which == 1 && A.gphase) {
@@ -525,34 +533,35 @@ __global__ __launch_bounds__(c2w::THREADS, 2) void k_fused2w(C2WArgs A) {
         const double th = which == 0 ? -(rpl * s * kf) : (aj * s * kf);
         double sn, cs;
         sincos(th, &sn, &cs);
-        val = cx<float>{(float)cs, (float)sn};
+        val = cx<T>{(T)cs, (T)sn};
       }
       (which ? tG : tE)[b][ax][k] = val;
     }
   };
-  cx<float> acc[32];
+  auto split_row = [&](cx<T> (&r)[H], const cx<T>& a0, const cx<T>& a1, int n) {
+    r[n] = cx<T>{a0.re + sg * a1.re, a0.im + sg * a1.im};
+  };
+  cx<T> acc[H];
 #pragma unroll
-  for (int k = 0; k < 32; ++k) acc[k] = cx<float>{0.f, 0.f};
+  for (int k = 0; k < H; ++k) acc[k] = cx<T>{(T)0, (T)0};
   load_tables(0, 0);
   __syncthreads();
-  cx<float> r[32];
+  cx<T> r[H];
   for (int mm = 0; mm < A.m; ++mm) {
     const int b = mm & 1;
     // A: row p = kx: E * fhat, inverse along axis 1 (ky -> y); the row factor
     // E_x(kx) is constant along the row and applied to the outputs
     {
-      const cx<float>* fr = fh + (long long)p * N;
+      const cx<T>* fr = fh + (long long)p * N;
 #pragma unroll
-      for (int n = 0; n < 32; ++n) {
-        const cx<float> a0 = tE[b][1][n] * fr[n];
-        const cx<float> a1 = tE[b][1][n + 32] * fr[n + 32];
-        r[n] = cx<float>{fmaf(sg, a1.re, a0.re), fmaf(sg, a1.im, a0.im)};
+      for (int n = 0; n < H; ++n) {
+        split_row(r, tE[b][1][n] * fr[n], tE[b][1][n + H] * fr[n + H], n);
         if ((n & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // 16 loads in flight at a time
       }
-      half_dif64<+1>(r, tw);
-      const cx<float> ex = tE[b][0][p];
+      half_dif<+1>(r, tw);
+      const cx<T> ex = tE[b][0][p];
 #pragma unroll
-      for (int k = 0; k < 32; ++k) W[p][2 * k + h] = ex * r[k];
+      for (int k = 0; k < H; ++k) W[p][2 * k + h] = ex * r[k];
     }
     __syncthreads();
     // B: column p = y: inverse along axis 0 (kx -> x), * f, forward along axis 0;
@@ -560,33 +569,27 @@ __global__ __launch_bounds__(c2w::THREADS, 2) void k_fused2w(C2WArgs A) {
     // the previous pair, are behind the barrier above)
     {
 #pragma unroll
-      for (int n = 0; n < 32; ++n) {
-        const cx<float> a0 = W[n][p], a1 = W[n + 32][p];
-        r[n] = cx<float>{fmaf(sg, a1.re, a0.re), fmaf(sg, a1.im, a0.im)};
-      }
-      half_dif64<+1>(r, tw);
-      // 32-bit offset made opaque per pair, so the 32 addresses are formed next to
+      for (int n = 0; n < H; ++n) split_row(r, W[n][p], W[n + H][p], n);
+      half_dif<+1>(r, tw);
+      // 32-bit offset made opaque per pair, so the addresses are formed next to
       // their loads instead of being held in registers across the pair loop
       int toff = h * N + p;
       asm volatile("" : "+v"(toff));
 #pragma unroll
-      for (int k = 0; k < 32; ++k) {
+      for (int k = 0; k < H; ++k) {
         r[k] = fc[2 * k * N + toff] * r[k];
         if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
       wave_sync2();  // both lanes of the column have read it
 #pragma unroll
-      for (int k = 0; k < 32; ++k) W[2 * k + h][p] = r[k];
+      for (int k = 0; k < H; ++k) W[2 * k + h][p] = r[k];
       wave_sync2();  // ... and written their halves
 #pragma unroll
-      for (int n = 0; n < 32; ++n) {
-        const cx<float> a0 = W[n][p], a1 = W[n + 32][p];
-        r[n] = cx<float>{fmaf(sg, a1.re, a0.re), fmaf(sg, a1.im, a0.im)};
-      }
-      half_dif64<-1>(r, tw);
+      for (int n = 0; n < H; ++n) split_row(r, W[n][p], W[n + H][p], n);
+      half_dif<-1>(r, tw);
       wave_sync2();
 #pragma unroll
-      for (int k = 0; k < 32; ++k) W[2 * k + h][p] = r[k];
+      for (int k = 0; k < H; ++k) W[2 * k + h][p] = r[k];
     }
     if (mm + 1 < A.m) load_tables(mm + 1, b ^ 1);
     __syncthreads();
@@ -595,58 +598,56 @@ __global__ __launch_bounds__(c2w::THREADS, 2) void k_fused2w(C2WArgs A) {
     // overwrite it without a barrier)
     {
 #pragma unroll
-      for (int n = 0; n < 32; ++n) {
-        const cx<float> a0 = W[p][n], a1 = W[p][n + 32];
-        r[n] = cx<float>{fmaf(sg, a1.re, a0.re), fmaf(sg, a1.im, a0.im)};
-      }
-      half_dif64<-1>(r, tw);
-      const cx<float> gx = (float)A.wsig[mm] * tG[b][0][p];
+      for (int n = 0; n < H; ++n) split_row(r, W[p][n], W[p][n + H], n);
+      half_dif<-1>(r, tw);
+      const cx<T> gx = (T)A.wsig[mm] * tG[b][0][p];
 #pragma unroll
-      for (int k = 0; k < 32; ++k) acc[k] = acc[k] + (gx * tG[b][1][2 * k + h]) * r[k];
+      for (int k = 0; k < H; ++k) acc[k] = acc[k] + (gx * tG[b][1][2 * k + h]) * r[k];
       wave_sync2();
     }
   }
   // radial weight and sp(a_j r_j |k|), then this (cell, j)'s partial spectrum
-  const float cj = (float)A.crad[ps * A.nr + j];
-  const float* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
+  const T cj = (T)A.crad[ps * A.nr + j];
+  const T* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
   const int kx = kfreq(p, N);
-  cx<float>* out = A.part + ((long long)cell * A.nr + j) * NPTS + (long long)p * N;
+  cx<T>* out = A.part + ((long long)cell * A.nr + j) * NPTS + (long long)p * N;
 #pragma unroll
-  for (int k = 0; k < 32; ++k) {
+  for (int k = 0; k < H; ++k) {
     const int ky = kfreq(2 * k + h, N);
     out[2 * k + h] = (cj * sp[kx * kx + ky * ky]) * acc[k];
   }
 }
 
-static int run_fused2w(Plan* P, const double* f, double* q, long long batch, const int* pidx,
-                       int with_loss, double hb, hipStream_t st) {
-  using namespace c2w;
+template <typename T, int N>
+static int run_fusedw(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                      int with_loss, double hb, hipStream_t st) {
+  constexpr long long NPTS = (long long)N * N;
   const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 1024);
   const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
   int rc;
   if ((rc = P->fhat.ensure(sub * b64))) return rc;
   if ((rc = P->acc.ensure(sub * b64))) return rc;
   if ((rc = P->work.ensure(2 * sub * b64))) return rc;
-  const size_t part_b = (size_t)sub * P->nr * NPTS * sizeof(cx<float>);
-  const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<float>);
-  if ((rc = P->aux.ensure(part_b + fh_b + (size_t)sub * NPTS * sizeof(float)))) return rc;
-  cx<float>* part = (cx<float>*)P->aux.ptr;
-  cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + part_b);
-  float* f32 = (float*)((char*)fh32 + fh_b);
+  const size_t part_b = (size_t)sub * P->nr * NPTS * sizeof(cx<T>);
+  const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<T>);
+  if ((rc = P->aux.ensure(part_b + fh_b + (size_t)sub * NPTS * sizeof(T)))) return rc;
+  cx<T>* part = (cx<T>*)P->aux.ptr;
+  cx<T>* fhn = (cx<T>*)((char*)P->aux.ptr + part_b);
+  T* fT = (T*)((char*)fhn + fh_b);
   for (long long c0 = 0; c0 < batch; c0 += sub) {
     const long long nb = std::min(sub, batch - c0);
     const double* fc = f + c0 * NPTS;
     const int* pc = pidx ? pidx + c0 : nullptr;
     if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
     const long long tot = nb * NPTS;
-    hipLaunchKernelGGL(k_f2_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                       (const cx<double>*)P->fhat.ptr, fc, fh32, f32, tot, 1.0 / (double)NPTS);
+    hipLaunchKernelGGL(k_f2_pack<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       (const cx<double>*)P->fhat.ptr, fc, fhn, fT, tot, 1.0 / (double)NPTS);
     IB_HIP(hipGetLastError());
-    C2WArgs A{};
-    A.fh = fh32;
-    A.f32 = f32;
-    A.ephase = (const cx<float>*)P->d_ephase;
-    A.gphase = (const cx<float>*)P->d_gphase;
+    CWArgs<T> A{};
+    A.fh = fhn;
+    A.f = fT;
+    A.ephase = (const cx<T>*)P->d_ephase;
+    A.gphase = (const cx<T>*)P->d_gphase;
     A.part = part;
     A.pidx = pc;
     A.batch = nb;
@@ -655,17 +656,17 @@ static int run_fused2w(Plan* P, const double* f, double* q, long long batch, con
     A.wsig = P->d_wsig;
     A.crad = P->d_crad;
     A.a = P->d_a;
-    A.spg = (const float*)P->d_spg;
+    A.spg = (const T*)P->d_spg;
     A.nr = P->nr;
     A.m = P->m;
     A.nq = P->nq;
     A.L = P->L;
     const long long groups = (nb + 7) / 8;
     if ((rc = P->timed_begin(st))) return rc;
-    hipLaunchKernelGGL(k_fused2w, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
+    hipLaunchKernelGGL((k_fusedw<T, N>), dim3((unsigned)(groups * 8 * P->nr)), dim3(2 * N), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
-    hipLaunchKernelGGL(k_f2_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
+    hipLaunchKernelGGL(k_f2_reduce<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
                        (cx<double>*)P->acc.ptr, nb, P->nr, NPTS);
     IB_HIP(hipGetLastError());
     if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
@@ -703,7 +704,7 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
     const int* pc = pidx ? pidx + c0 : nullptr;
     if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
     const long long tot = nb * NPTS;
-    hipLaunchKernelGGL(k_f2_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_f2_pack<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                        (const cx<double>*)P->fhat.ptr, fc, fh32, f32, tot, 1.0 / (double)NPTS);
     IB_HIP(hipGetLastError());
     F2Args A{};
@@ -729,7 +730,7 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
     hipLaunchKernelGGL(k_fused2<N>, dim3((unsigned)(nb * wpc)), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
-    hipLaunchKernelGGL(k_f2_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
+    hipLaunchKernelGGL(k_f2_reduce<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
                        (cx<double>*)P->acc.ptr, nb, wpc, NPTS);
     IB_HIP(hipGetLastError());
     if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
@@ -742,6 +743,11 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
 int collide_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                    int with_loss, double hb, hipStream_t st, int* handled) {
   *handled = 1;
+  if (P->precision == IB_PREC_FP64) {  // FP64 fused form of 2D 32^2 (FP64 throughout)
+    if (P->dim == 2 && P->n == 32) return run_fusedw<double, 32>(P, f, q, batch, pidx, with_loss, hb, st);
+    *handled = 0;
+    return IB_OK;
+  }
   if (P->dim == 3) {
     if (P->n == 16) return run_fusedc3(P, f, q, batch, pidx, with_loss, hb, st);
     *handled = 0;
@@ -750,7 +756,7 @@ int collide_fused2(Plan* P, const double* f, double* q, long long batch, const i
   switch (P->n) {
     case 16: return run_fused2<16>(P, f, q, batch, pidx, with_loss, hb, st);
     case 32: return run_fused2<32>(P, f, q, batch, pidx, with_loss, hb, st);
-    case 64: return run_fused2w(P, f, q, batch, pidx, with_loss, hb, st);
+    case 64: return run_fusedw<float, 64>(P, f, q, batch, pidx, with_loss, hb, st);
     default:
       *handled = 0;
       return IB_OK;

@@ -317,6 +317,29 @@ def test_fused_fp32_2d_many_parameter_sets(pkg, need_gpu, N, nr):
     assert worst <= TOL32, worst
 
 
+def test_fused_fp64_2d32_many_parameter_sets_and_sub_batches(pkg, need_gpu):
+    """The FP64 fused form of 2D 32^2 (two lanes per pencil): 70 cells with their own e and
+    gamma (more parameter sets than the phase tables hold, so the kernel evaluates G itself),
+    in sub-batches of 24 cells, against the FP64 oracle at the 1e-12 bar."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    d, N, nr, M, s = 2, 32, 16, 8, 3.624654714576
+    cfg = _cfg(d, N, nr, M, s, 1.0, 1.0, 1.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm)
+    coll.set_tuning(fused_cells=24)
+    es = np.linspace(0.2, 1.0, 70)
+    gs = (np.arange(70) % 2).astype(float)
+    f = _data(d, N, sm.L, 70, seed=71)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True, e=es, gamma=gs).cpu().numpy()
+    worst = 0.0
+    for c in (0, 23, 24, 47, 69):
+        ref = _oracle(d, N, nr, M, s, es[c], es[c], gs[c]).collide(f[c], with_loss=True)
+        worst = max(worst, _err(q[c], ref))
+    print("fused FP64 2D32 worst rel Linf", worst)
+    assert worst <= TOL64, worst
+
+
 @pytest.mark.parametrize("d,N,nr,M,s", [(3, 32, 8, 14, 3.624654714576), (2, 32, 16, 8, 3.0),
                                        (3, 16, 16, 38, 3.0), (2, 64, 16, 8, 3.0)])
 def test_fused_fp32_batch_invariance_full_size(pkg, need_gpu, d, N, nr, M, s):
