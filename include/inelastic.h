@@ -28,7 +28,8 @@ extern "C" {
 #define IB_ERR_UNSUPPORTED -3 /* NotImplementedError in the reference */
 #define IB_ERR_BC -4        /* missing boundary condition (Exception) */
 
-#define IB_PREC_FP64 0      /* plain FP64 form: C2C, staged batched FFTs        */
+#define IB_PREC_FP64 0      /* plain FP64 form: C2C, staged batched FFTs (2D 32^2:
+                               a fused FP64 kernel), FP64 throughout            */
 #define IB_PREC_FP32 1      /* FP32 fast form: FP32 transforms and angular sums,
                                FP64 radial sums, final transform and loss term;
                                fused kernels for 3D 32^3 and 16^3 and 2D 16^2 / 32^2 / 64^2 */
