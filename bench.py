@@ -494,7 +494,7 @@ SIDE_SHAPES = {
     # name: (dim, N, Nr, M, s, fused kernel, what)
     "2d32": (2, 32, 32, 8, S3, "k_fused2",
              "2D 32^2, Nr=32, M=8 (the homogeneous-cooling and 1D x 2D shape)"),
-    "2d64": (2, 64, 64, 8, 3.0, "k_fused2w",
+    "2d64": (2, 64, 64, 8, 3.0, "k_fusedw",
              "2D 64^2, Nr=64, M=8, s=3 (the 2D 64^2 shape of the uneven batch)"),
     "3d16": (3, 16, 16, 38, 3.0, "k_fusedc3",
              "3D 16^3, Nr=16, M=38 Lebedev, s=3 (the 3D shape of the uneven batch and serving mix)"),
