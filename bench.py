@@ -15,6 +15,14 @@ the dominant kernel of the FP32 form (VALU-bound FFT work, algorithmic flops
 per cell-eval = (2 Nr M + Nr + 2) 5 n log2 n + (16 Nr M + 7 Nr) n).
 ``cpu_baseline`` times the FP64 numpy/scipy oracle (8 FFT worker threads) on a
 bounded sample of the same workload.
+
+Side measurements in the same JSON line: the Kahan-compensated FP32 form, the
+sustained FMA rate of the vector ALU (``valu_fma_peak``), the HBM-bound kernels of
+the step loop at large sizes (``hbm_kernels``), three more grid shapes with the
+roofline of their fused kernels (``shape_2d_32``, ``shape_3d_16``,
+``shape_2d_64``), the reference's two 2D configurations through the solver API
+(``reference_configs_fp64``), the uneven batch of homogeneous cooling problems
+and the Poisson serving loop (p50/p99).  Each has a switch to skip it.
 """
 import argparse
 import json
