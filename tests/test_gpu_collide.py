@@ -452,3 +452,18 @@ def test_autotune_picks_a_candidate_and_keeps_results(pkg, need_gpu, tmp_path, m
         best = autotune.tune(tuned, f)
         assert best in autotune.candidates(tuned, 70)
     assert (tmp_path / "tune.json").exists()
+
+
+@pytest.mark.parametrize("prec,d,N", [("fp64", 2, 32), ("fp32", 3, 32), ("fp32", 3, 16), ("fp32", 2, 64)])
+def test_empty_batch(pkg, need_gpu, prec, d, N):
+    """A batch of zero cells is a no-op on every path (no launch with an empty grid)."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    cfg = _cfg(d, N, 4, 8 if d == 2 else 6, 3.0, 0.9, 0.9, 1.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision=prec)
+    f = torch.empty((0,) + (N,) * d, dtype=torch.float64, device="cuda")
+    q = coll.collide(f, with_loss=True, project=True)
+    assert tuple(q.shape) == tuple(f.shape)
+    assert tuple(coll.moments(f).shape) == (0, d + 2)
+    torch.cuda.synchronize()
