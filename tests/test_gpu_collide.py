@@ -467,3 +467,34 @@ def test_empty_batch(pkg, need_gpu, prec, d, N):
     assert tuple(q.shape) == tuple(f.shape)
     assert tuple(coll.moments(f).shape) == (0, d + 2)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+@pytest.mark.parametrize("prec,d,N,nr,M", [("fp32", 2, 16, 8, 8), ("fp32", 2, 32, 8, 8), ("fp32", 2, 64, 8, 8),
+                                          ("fp32", 3, 16, 6, 14), ("fp32", 3, 32, 4, 6), ("fp64", 2, 32, 8, 8)])
+def test_fused_random_configurations(pkg, need_gpu, seed, prec, d, N, nr, M):
+    """Every fused form with a random plan-level restitution profile (e != e0, the tanh blend),
+    kernel exponent, heat bath and gain-only or full Q, on 3 cells, against the oracle."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    rng = np.random.default_rng(seed * 100 + d * 10 + N)
+    e, e0 = rng.uniform(0.2, 1.0, 2)
+    g = float(rng.integers(0, 2))
+    hb = float(rng.choice([0.0, 0.03]))
+    with_loss = bool(rng.integers(0, 2))
+    s = 3.624654714576 if (d == 3 and N == 32) else 3.0
+    cfg = _cfg(d, N, nr, M, s, e, e0, g)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision=prec)
+    op = _oracle(d, N, nr, M, s, e, e0, g)
+    f = _smooth(d, N, sm.L, 3, seed=seed) if (d == 3 and N == 32) else _data(d, N, sm.L, 3, seed=seed)
+    if with_loss:
+        q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True, heat_bath=hb).cpu().numpy()
+        refs = [op.collide(f[c], with_loss=True) + (hb * op.laplacian(f[c]) if hb else 0) for c in range(3)]
+    else:
+        q = coll(torch.from_numpy(f).cuda(), heat_bath=hb).cpu().numpy()
+        refs = [op(f[c], heat_bath=hb) for c in range(3)]
+    tol = TOL64 if prec == "fp64" else TOL32
+    worst = max(_err(q[c], refs[c]) for c in range(3))
+    print(prec, d, N, "e", e, "e0", e0, "g", g, "hb", hb, "loss", with_loss, "worst", worst)
+    assert worst <= tol, worst
