@@ -271,8 +271,11 @@ struct C3Args {
   double L;
 };
 
-// Pair schedule (4 workgroup barriers per quadrature pair); W[x][y][z] rows are
-// padded to N + 1, so every pass reads and writes 16 lanes of distinct banks.
+// Pair schedule (2 workgroup barriers per quadrature pair): plane kx of the work
+// array is written and read in passes A, B, D and E only by the 16 threads (kx, *),
+// which share a wave, so those exchanges need no barrier; the x pass C reads and
+// writes every plane.  W[x][y][z] rows are padded to N + 1, so every pass reads
+// and writes 16 lanes of distinct banks.
 __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
   using namespace c3;
   __shared__ cx<float> W[N][N][PAD];
@@ -331,9 +334,8 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
 #pragma unroll
       for (int z = 0; z < N; ++z) W[r0][r1][z] = v[z];
     }
-    __syncthreads();
-    // B: inverse along y at (kx, z) = (r0, r1); the next pair's phase tables (the
-    // other buffer: its last reader, the previous pair, is behind the barrier above)
+    wave_sync2();  // plane kx = r0 belongs to the 16 threads (r0, *) of one wave
+    // B: inverse along y at (kx, z) = (r0, r1)
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const int k = bitrev<N>(i);
@@ -342,7 +344,6 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
     fft_reg<N, +1>(v);
 #pragma unroll
     for (int y = 0; y < N; ++y) W[r0][y][r1] = v[y];
-    if (mm + 1 < A.m) make_tables(mm + 1, b ^ 1);
     __syncthreads();
     // C: along x at (y, z) = (r0, r1): inverse, * f, forward
 #pragma unroll
@@ -356,6 +357,9 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
     fft_reg<N, -1>(v);
 #pragma unroll
     for (int k = 0; k < N; ++k) W[k][r0][r1] = v[k];
+    // the next pair's phase tables, into the other buffer: its last readers (phases A
+    // and E of the previous pair) are behind the barrier above
+    if (mm + 1 < A.m) make_tables(mm + 1, b ^ 1);
     __syncthreads();
     // D: forward along y at (kx, z) = (r0, r1)
 #pragma unroll
@@ -366,7 +370,7 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
     fft_reg<N, -1>(v);
 #pragma unroll
     for (int k = 0; k < N; ++k) W[r0][k][r1] = v[k];
-    __syncthreads();
+    wave_sync2();  // again within the plane's wave
     // E: forward along z at (kx, ky) = (r0, r1), G * (.) and the sum over angles
     // (the thread's own row: the next pair's phase A may overwrite it right after)
 #pragma unroll
