@@ -173,7 +173,7 @@ def serve_run(precision, nreq, rate, seed=5):
     1D jobs: Nx in {64, 128}, 32^2 velocity, eps = 1e-2, hard spheres,
     e ~ U[0.5, 1], Sod-type Maxwellian Riemann data, PFE (2 inner steps of
     2.5e-4, CFL <= 0.26), dt = 1e-3, t_final in {0.01, 0.02}.  Returns latency
-    statistics.
+    statistics and the results.
     """
     from proj_int_inelastic_amd import serving
     rng = np.random.default_rng(seed)
@@ -215,7 +215,7 @@ def serve_run(precision, nreq, rate, seed=5):
                                         arrival=float(t_arr)))
     srv = serving.Server(hbm_budget=32 << 30)
     res = srv.run(reqs, realtime=True)
-    return serving.latency_stats(res)
+    return serving.latency_stats(res), res
 
 
 def serve_sweep_requests(precision, nreq, rate, seed):
@@ -761,13 +761,16 @@ def main():
             errs = [np.abs(r32[k].f - r64[k].f).max() / np.abs(r64[k].f).max() for k in r64]
             out["uneven_batch"]["fp32_vs_fp64_final_f_rel_linf_max"] = float(max(errs))
     if args.serve_requests > 0:
-        st32 = serve_run("fp32", args.serve_requests, args.serve_rate)
+        st32, sr32 = serve_run("fp32", args.serve_requests, args.serve_rate)
         out["serving"] = {"fp32": st32,
                           "workload": "Poisson arrivals (rate %.0f/s) of %d solves: homogeneous 2D 16^2/32^2, "
                                       "3D 16^3 TPI2 cooling + 1D x 2D PFE jobs" % (args.serve_rate,
                                                                                    args.serve_requests)}
         if not args.no_fp64:
-            out["serving"]["fp64"] = serve_run("fp64", args.serve_requests, args.serve_rate)
+            st64, sr64 = serve_run("fp64", args.serve_requests, args.serve_rate)
+            out["serving"]["fp64"] = st64
+            errs = [np.abs(sr32[k].f - sr64[k].f).max() / np.abs(sr64[k].f).max() for k in sr64]
+            out["serving"]["fp32_vs_fp64_final_f_rel_linf_max"] = float(max(errs))
         out["p99_solve_latency_s"] = st32["p99_s"]
     if not args.no_cpu:
         rates, ncpu, model = cpu_baseline(args.cpu_cells, seed=1)
