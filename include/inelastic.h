@@ -90,7 +90,8 @@ int ib_plan_set_timing(ib_plan* plan, int enable);
 int ib_plan_get_timing(ib_plan* plan, double* ms_total, long long* launches, long long* units);
 
 /* Q(f, f) for `batch` cells: f, q are (batch, n^dim) float64 device arrays.
- * param_idx: device int32 (batch,) parameter-set index per cell, or NULL for 0.
+ * param_idx: device int32 (batch,) parameter-set index per cell, or NULL for 0;
+ * every entry must lie in [0, n_param) (device-resident, so not checked here).
  * with_loss = 0 returns the gain term only (the reference default);
  * heat_bath != 0 adds heat_bath * Laplacian(f).
  * Replaces src/collision/base.py:32-58 (__call__) and

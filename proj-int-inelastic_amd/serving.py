@@ -298,6 +298,19 @@ class Server:
         self.t0 = None
 
     def submit(self, req):
+        """Queue a request.  Requests that no state of the server could admit raise
+        here instead of waiting forever: a memory need above the whole budget, more
+        cells than a class may hold, or a step that does not divide t_final (the
+        reference's evolve_to_time check, src/solver.py:122-128)."""
+        if req.nbytes() > self.budget:
+            raise ValueError("request %r needs %d bytes, above the HBM budget of %d"
+                             % (req.rid, req.nbytes(), self.budget))
+        if self.max_cells is not None and req.cells > self.max_cells:
+            raise ValueError("request %r has %d cells, above max_cells_per_class = %d"
+                             % (req.rid, req.cells, self.max_cells))
+        n = int(round(req.t_final / req.dt))
+        if abs(n * req.dt - req.t_final) > 1e-5 * req.t_final:
+            raise Exception("dt does not divide (tend-tstart) and dt is fixed!")
         self.queue.append(req)
 
     def _now(self):
@@ -322,9 +335,7 @@ class Server:
                 waiting.append(r)
                 continue
             cells[key] += r.cells
-            n = int(round(r.t_final / r.dt))
-            if abs(n * r.dt - r.t_final) > 1e-5 * r.t_final:
-                raise Exception("dt does not divide (tend-tstart) and dt is fixed!")
+            n = int(round(r.t_final / r.dt))  # submit() checked that dt divides t_final
             a = _Active(req=r, start=0, cells=r.cells, steps_left=n, admitted_at=now)
             admitted.setdefault(r.class_key(), []).append((a, r.f0))
             self.used += r.nbytes()

@@ -132,6 +132,26 @@ def test_python_layer_rejects_before_the_device(pkg):
         s._bc_codes(0)
 
 
+def test_server_rejects_requests_it_could_never_admit(pkg):
+    """A request above the whole HBM budget or the per-class cell cap, or whose dt does
+    not divide t_final, raises at submit instead of waiting in the queue forever."""
+    from proj_int_inelastic_amd import serving
+    f0 = np.zeros((16, 16))
+    ok = serving.Request(rid=0, f0=f0, t_final=0.5, e=0.9, n=16, nr=16, dt=0.25)
+    job = serving.Request(rid=1, f0=np.zeros((4, 16, 16)), t_final=0.5, e=0.9, n=16, nr=16,
+                          dt=0.25, xlen=1.0)
+    srv = serving.Server(hbm_budget=job.nbytes(), max_cells_per_class=1)
+    srv.submit(ok)
+    assert len(srv.queue) == 1
+    with pytest.raises(ValueError, match="HBM budget"):
+        serving.Server(hbm_budget=ok.nbytes() - 1).submit(ok)
+    with pytest.raises(ValueError, match="max_cells_per_class"):
+        srv.submit(job)
+    with pytest.raises(Exception, match="dt does not divide"):
+        srv.submit(serving.Request(rid=2, f0=f0, t_final=0.6, e=0.9, n=16, nr=16, dt=0.25))
+    assert [r.rid for r in srv.queue] == [0]
+
+
 def test_probe_library_rejects_bad_arguments():
     """The FMA-throughput probe the benchmark loads validates before any HIP call."""
     import pathlib
