@@ -89,7 +89,6 @@ class BoltzmannSolver(Solver):
             fn(solution, take_one_step, tstart, tend)  # eager warm-up: buffers sized
             t.cuda.synchronize()
             g = t.cuda.CUDAGraph()
-            self._graph_f0 = state.f.clone()
             with t.cuda.graph(g):
                 fn(solution, take_one_step, tstart, tend)
             # capture does not execute; the warm-up already advanced f by one step

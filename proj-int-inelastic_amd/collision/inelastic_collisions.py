@@ -33,6 +33,12 @@ from .. import device as dev
 from .base import BaseCollision
 
 
+def _per_cell(x, nb):
+    """A scalar, a (nb,) array or an array of the batch shape -> (nb,) float64."""
+    a = np.asarray(x, dtype=np.float64)
+    return a.reshape(nb) if a.size == nb else np.broadcast_to(a, (nb,))
+
+
 class FSInelasticVHSCollision(BaseCollision):
 
     collision_model = "vhs"
@@ -145,9 +151,9 @@ class FSInelasticVHSCollision(BaseCollision):
             return None
         if e is None:
             e, e0 = self._e, self._e0
-        e = np.broadcast_to(np.asarray(e, dtype=np.float64), (nb,))
-        e0 = e if e0 is None else np.broadcast_to(np.asarray(e0, dtype=np.float64), (nb,))
-        g = np.broadcast_to(np.asarray(self._gamma if gamma is None else gamma, dtype=np.float64), (nb,))
+        e = _per_cell(e, nb)
+        e0 = e if e0 is None else _per_cell(e0, nb)
+        g = _per_cell(self._gamma if gamma is None else gamma, nb)
         triples = np.stack([e, e0, g], axis=1)
         cache = getattr(self, "_pidx_cache", None)
         tkey = triples.tobytes()

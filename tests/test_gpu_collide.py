@@ -158,6 +158,10 @@ def test_per_cell_restitution(pkg, need_gpu, prec, tol):
     for c in range(4):
         ref = _oracle(d, N, nr, M, s, es[c], es[c], 1.0).collide(f[c], with_loss=True)
         assert _err(q[c], ref) <= tol, (c, _err(q[c], ref))
+    # per-cell arrays may carry the batch shape of f
+    q2 = coll.collide(torch.from_numpy(f.reshape((2, 2) + f.shape[1:])).cuda(), with_loss=True,
+                      e=es.reshape(2, 2)).cpu().numpy()
+    assert np.array_equal(q2.reshape(q.shape), q)
     # back to the configured restitution
     q0 = coll.collide(torch.from_numpy(f[:1]).cuda(), with_loss=True).cpu().numpy()
     ref0 = _oracle(d, N, nr, M, s, 1.0, 1.0, 1.0).collide(f[:1], with_loss=True)
