@@ -153,7 +153,8 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   for (int q = tid; q < A.nq; q += THREADS) sp[q] = A.spg[((long long)ps * A.nr + j) * A.nq + q];
   const double aj = A.a[ps * A.nr + j] * A.rj[j];
   const double crad_j = A.crad[ps * A.nr + j];
-  const double rpl = M_PI * A.rj[j] / A.L;
+  const double rpl_pi = A.rj[j] / A.L;       // pi r_j / L in half turns
+  const double aj_pi = aj / M_PI;
   const int p = tid / N, l = tid % N;                 // plane-pass role
   const cx<float>* fh = A.fh + (A.cell0 + cell) * 17 * PLANE;
   const float* fc = A.f32 + (A.cell0 + cell) * NPTS;
@@ -177,12 +178,19 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       const int which = tq / (3 * N), ax = (tq % (3 * N)) / N, k = tq % N;
       const double s = A.sigma[mm * 3 + ax];
       const double kf = (double)kfreq(k, N);
-      const double th = which == 0 ? -(rpl * s * kf) : (aj * s * kf);
-      double sn, cs;
-      sincos(th, &sn, &cs);
-      (which ? tG : tE)[b][ax][k] = cx<float>{(float)cs, (float)sn};
+      // the angle in half turns, reduced to [-1, 1] in FP64; the FP32 sincospi of the
+      // reduced value is as accurate as the FP32 rounding of an FP64 sincos
+      double t = which == 0 ? -(rpl_pi * s * kf) : (aj_pi * s * kf);
+      t -= 2.0 * rint(0.5 * t);
+      float sn, cs;
+      sincospif((float)t, &sn, &cs);
+      (which ? tG : tE)[b][ax][k] = cx<float>{cs, sn};
     }
   };
+  // fhat on the kx = 16 plane at this thread's two points (the same for every pair)
+  cx<float> f16[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) f16[h] = fh[16 * PLANE + tid + h * THREADS];
   auto make_x16 = [&](int b) {
     int tq = tid;
     asm volatile("" : "+v"(tq));
@@ -190,7 +198,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     for (int h = 0; h < 2; ++h) {
       const int e = tq + h * THREADS, ky = e / N, kz = e % N;
       const SymRow r16 = sym_row(tE[b][0][N / 2], true, tE[b][1][ky], ky == N / 2);
-      S16[ky][kz] = sym_at(r16, tE[b][2][kz], kz == N / 2) * fh[16 * PLANE + e];
+      S16[ky][kz] = sym_at(r16, tE[b][2][kz], kz == N / 2) * f16[h];
     }
   };
   auto packed_acc = [&](int b, float wjp) {
