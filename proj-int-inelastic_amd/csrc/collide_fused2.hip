@@ -301,8 +301,8 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
   if (cell >= A.batch) return;  // uniform over the workgroup
   const int ps = A.pidx ? A.pidx[cell] : 0;
   for (int q = tid; q < A.nq; q += THREADS) sp[q] = A.spg[((long long)ps * A.nr + j) * A.nq + q];
-  const double aj = A.a[ps * A.nr + j] * A.rj[j];
-  const double rpl = M_PI * A.rj[j] / A.L;
+  const double aj_pi = A.a[ps * A.nr + j] * A.rj[j] / M_PI;
+  const double rpl_pi = A.rj[j] / A.L;  // pi r_j / L in half turns
   // pencil role: the two fixed indices (r0, r1) of the thread's pencil in every pass
   const int r0 = tid / N, r1 = tid % N;
   cx<float> fhv[N];  // fhat along z at (kx, ky) = (r0, r1)
@@ -319,16 +319,19 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
 #pragma unroll
   for (int k = 0; k < N; ++k) acc[k] = cx<float>{0.f, 0.f};
   // phase factors of pair mm along each axis (the separable factors of
-  // inelastic_collisions.py:97-105), FP64 sincos rounded to FP32
+  // inelastic_collisions.py:97-105), FP32 sincospi of the FP64-reduced angle
   auto make_tables = [&](int mm, int b) {
     if (tid < 6 * N) {
       const int which = tid / (3 * N), ax = (tid % (3 * N)) / N, k = tid % N;
       const double s = A.sigma[mm * 3 + ax];
       const double kf = (double)kfreq(k, N);
-      const double th = which == 0 ? -(rpl * s * kf) : (aj * s * kf);
-      double sn, cs;
-      sincos(th, &sn, &cs);
-      (which ? tG : tE)[b][ax][k] = cx<float>{(float)cs, (float)sn};
+      // the angle in half turns, reduced to [-1, 1] in FP64; the FP32 sincospi of the
+      // reduced value is as accurate as the FP32 rounding of an FP64 sincos
+      double t = which == 0 ? -(rpl_pi * s * kf) : (aj_pi * s * kf);
+      t -= 2.0 * rint(0.5 * t);
+      float sn, cs;
+      sincospif((float)t, &sn, &cs);
+      (which ? tG : tE)[b][ax][k] = cx<float>{cs, sn};
     }
   };
   make_tables(0, 0);
