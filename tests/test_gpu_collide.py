@@ -502,3 +502,22 @@ def test_fused_random_configurations(pkg, need_gpu, seed, prec, d, N, nr, M):
     worst = max(_err(q[c], refs[c]) for c in range(3))
     print(prec, d, N, "e", e, "e0", e0, "g", g, "hb", hb, "loss", with_loss, "worst", worst)
     assert worst <= tol, worst
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("d,N,nr,M", [(2, 16, 1, 1), (2, 32, 1, 1), (2, 64, 1, 2), (3, 16, 1, 6),
+                                      (3, 32, 1, 6)])
+def test_single_radial_node_and_angle(pkg, need_gpu, prec, d, N, nr, M):
+    """Nr = 1 and M = 1 (the reference's squeeze() collapses these axes, quirk Q7; the device plan
+    keeps them), through every engine: staged FP64, the fused FP32 forms and the fused FP64 2D 32^2."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    s = 3.624654714576 if (d == 3 and N == 32) else 3.0
+    cfg = _cfg(d, N, nr, M, s, 0.7, 0.7, 1.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision=prec)
+    op = _oracle(d, N, nr, M, s, 0.7, 0.7, 1.0)
+    f = _smooth(d, N, sm.L, 2, seed=4) if (d == 3 and N == 32) else _data(d, N, sm.L, 2, seed=4)
+    q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True).cpu().numpy()
+    worst = max(_err(q[c], op.collide(f[c], with_loss=True)) for c in range(2))
+    assert worst <= (TOL64 if prec == "fp64" else TOL32), worst
