@@ -102,9 +102,11 @@ int ib_collide(ib_plan* plan, const double* f, double* q, long long batch,
 /* Grouped form of ib_collide for an uneven batch: nplans plans (any grid shapes,
  * one precision), each with its own cells f[i], q[i], batch[i], param_idx[i]
  * (param_idx may be NULL).  Every stage of the staged engine runs as ONE persistent
- * kernel launch over the work of all plans (up to 8 plans per launch); FP32 plans
- * whose shape has a fused kernel (3D 32^3 and 16^3, 2D 16^2, 32^2 and 64^2) run it.  New: the reference evaluates
- * one operator per call (src/collision/base.py:32-58). */
+ * kernel launch over the work of all plans (up to 8 plans per launch); plans
+ * whose shape has a fused kernel (FP32 3D 32^3 and 16^3, 2D 16^2, 32^2 and 64^2;
+ * FP64 2D 32^2) run it, and a plan whose batch is too large for chunks of 32
+ * quadrature pairs within its scratch budget runs alone in sub-batches.  New: the
+ * reference evaluates one operator per call (src/collision/base.py:32-58). */
 int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, double* const* q,
                      const long long* batch, const int* const* param_idx, int with_loss,
                      double heat_bath, void* stream);

@@ -62,6 +62,7 @@ void DevBuf::release() {
 
 int collide_plain(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                   int with_loss, double hb, int no_gain, hipStream_t st);
+bool staged_runs_alone(const Plan* P, long long batch);
 int collide_fused(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                   int with_loss, double hb, hipStream_t st, int* handled);
 int launch_tables(Plan* P, const double* d_wr, hipStream_t st);
@@ -349,6 +350,11 @@ int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, 
       int rc = collide_fused(p, f[i], q[i], batch[i], pi, with_loss, heat_bath, st, &handled);
       if (rc) return rc;
       if (handled) continue;
+    }
+    if (staged_runs_alone(p, batch[i])) {  // large batch: the plan's own sub-batches
+      int rc = collide_plain(p, f[i], q[i], batch[i], pi, with_loss, heat_bath, 0, st);
+      if (rc) return rc;
+      continue;
     }
     gp.push_back(p);
     gf.push_back(f[i]);

@@ -138,29 +138,27 @@ __device__ __forceinline__ void fft_reg(cx<T>* a) {
       for (int k = 0; k < half; ++k) {
         const cx<T> u = t[i + k];
         const int ti = k * step;
-        if constexpr (sizeof(T) == 4) {
-          if (ti % 16 != 0) {
-            // FP32: u + w v by FMAs, then u - w v = 2 u - (u + w v): 6 operations
-            // per butterfly instead of 8 (Linzer & Feig 1993)
-            const cx<T> x = t[i + k + half];
-            cx<T> a;
-            if (ti == 8 || ti == 24 || ti == 40 || ti == 56) {
-              const T h = (T)0.70710678118654752440;
-              const cx<T> w = twiddle<DIR>(cx<T>{(T)1, (T)0}, ti);
-              // w = h (w.re/h + i w.im/h) with components +-1
-              const T sr = w.re > 0 ? (T)1 : (T)-1, si = w.im > 0 ? (T)1 : (T)-1;
-              a.re = fmaf(h, sr * x.re - si * x.im, u.re);
-              a.im = fmaf(h, si * x.re + sr * x.im, u.im);
-            } else {
-              const T c = (T)TW64[ti][0];
-              const T s = (T)(DIR > 0 ? TW64[ti][1] : -TW64[ti][1]);
-              a.re = fmaf(x.re, c, fmaf(-x.im, s, u.re));
-              a.im = fmaf(x.re, s, fmaf(x.im, c, u.im));
-            }
-            t[i + k] = a;
-            t[i + k + half] = cx<T>{fmaf((T)2, u.re, -a.re), fmaf((T)2, u.im, -a.im)};
-            continue;
+        if (ti % 16 != 0) {
+          // u + w v by FMAs, then u - w v = 2 u - (u + w v): 6 operations per
+          // butterfly instead of 8 (Linzer & Feig 1993)
+          const cx<T> x = t[i + k + half];
+          cx<T> a;
+          if (ti == 8 || ti == 24 || ti == 40 || ti == 56) {
+            const T h = (T)0.70710678118654752440;
+            const cx<T> w = twiddle<DIR>(cx<T>{(T)1, (T)0}, ti);
+            // w = h (w.re/h + i w.im/h) with components +-1
+            const T sr = w.re > 0 ? (T)1 : (T)-1, si = w.im > 0 ? (T)1 : (T)-1;
+            a.re = fma(h, sr * x.re - si * x.im, u.re);
+            a.im = fma(h, si * x.re + sr * x.im, u.im);
+          } else {
+            const T c = (T)TW64[ti][0];
+            const T s = (T)(DIR > 0 ? TW64[ti][1] : -TW64[ti][1]);
+            a.re = fma(x.re, c, fma(-x.im, s, u.re));
+            a.im = fma(x.re, s, fma(x.im, c, u.im));
           }
+          t[i + k] = a;
+          t[i + k + half] = cx<T>{fma((T)2, u.re, -a.re), fma((T)2, u.im, -a.im)};
+          continue;
         }
         const cx<T> v = twiddle<DIR>(t[i + k + half], ti);
         t[i + k] = u + v;
