@@ -34,7 +34,8 @@ extern "C" {
                                FP64 radial sums, final transform and loss term;
                                fused kernels for 3D 32^3 and 16^3 and 2D 16^2 / 32^2 / 64^2 */
 #define IB_PREC_FP32_KAHAN 2 /* FP32 fast form whose angular sums in the fused
-                               3D 32^3 kernel are Kahan-compensated (other
+                               3D 32^3 kernel are Kahan-compensated, the
+                               compensation kept between angles as bf16 (other
                                shapes: as IB_PREC_FP32) */
 
 typedef struct ib_plan ib_plan;

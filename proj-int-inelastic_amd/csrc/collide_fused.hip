@@ -29,9 +29,9 @@
 #include "plan.h"
 
 // The sum over the M angles inside a workgroup is plain FP32 (plan precision
-// IB_PREC_FP32) or Kahan-compensated FP32 (IB_PREC_FP32_KAHAN: 64 more
-// registers per thread, which spill); the sum over radial nodes is FP64 either
-// way (k_f3_reduce).
+// IB_PREC_FP32) or Kahan-compensated FP32 (IB_PREC_FP32_KAHAN, compensation kept
+// between pairs as bf16 in an L2-resident buffer); the sum over radial nodes is
+// FP64 either way (k_f3_reduce).
 
 namespace ib {
 
@@ -87,6 +87,19 @@ __device__ __forceinline__ void acc_add(cx<float>& s, cx<float>& c, cx<float> x)
     (void)c;
     s = s + x;
   }
+}
+
+// A complex Kahan compensation kept between pairs as two bf16 halves of one
+// 32-bit word: the high 16 bits of each FP32 component (truncation).  The term is
+// the rounding error of the running sum, so dropping 2^-8 of it per step leaves
+// the compensated sum's error near (2 + M / 2^8) ulp instead of the plain sum's
+// (M - 1) ulp.
+__device__ __forceinline__ unsigned pack_bf16x2(cx<float> c) {
+  // bytes 3, 2 of c.im over bytes 3, 2 of c.re
+  return __builtin_amdgcn_perm(__float_as_uint(c.im), __float_as_uint(c.re), 0x07060302u);
+}
+__device__ __forceinline__ cx<float> unpack_bf16x2(unsigned b) {
+  return cx<float>{__uint_as_float(b << 16), __uint_as_float(b & 0xffff0000u)};
 }
 
 // Hermitian part of a separable factor phi_x(kx) phi_y(ky) phi_z(kz): every
@@ -159,14 +172,20 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   const cx<float>* fh = A.fh + (A.cell0 + cell) * 17 * PLANE;
   const float* fc = A.f32 + (A.cell0 + cell) * NPTS;
   const int kyf = kfreq(l, N);
+  // Kahan: the compensation of the kz-pencil sums (bf16 pairs) lives in this
+  // workgroup's compensation plane of the partial buffer, laid out [k][thread] so
+  // that every access of a wave is one contiguous 256-byte run, read and rewritten
+  // once per pair in phase E: 64 KiB per workgroup stay in L2, and the 64
+  // registers a register-held FP32 compensation needs stay out of the pair loop
+  // (register-held, it spilled 444 bytes per lane and ran at 5.5k cell-evals/s;
+  // FP32 in this buffer, 6.8k; bf16, 8.9k)
+  unsigned* compb =
+      (unsigned*)(A.part + ((A.cell0 + cell) * A.nr + j) * KP * NPL * PLANE + NPL * PLANE);
 
   cx<float> acc[N];
   cx<float> acc2[2], cmp2[2];
-  cx<float> cmp[KAHAN ? N : 1];
 #pragma unroll
   for (int k = 0; k < N; ++k) acc[k] = cx<float>{0.f, 0.f};
-#pragma unroll
-  for (int k = 0; k < (int)(sizeof(cmp) / sizeof(cmp[0])); ++k) cmp[k] = cx<float>{0.f, 0.f};
   acc2[0] = acc2[1] = cmp2[0] = cmp2[1] = cx<float>{0.f, 0.f};
 
   // The side tasks recompute their indices from an opaque copy of tid each pair,
@@ -349,7 +368,16 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       const int k = bitrev<N>(i);
       v[k] = W[p][l][k];
     }
+    unsigned cv[KAHAN ? N : 1];
+    int to = tid;  // Kahan: opaque per pair, so the 32 addresses are formed next to their accesses
+    if constexpr (KAHAN) asm volatile("" : "+v"(to));
     fft_reg<N, -1>(v);
+    if constexpr (KAHAN) {  // loaded after the transform (fewer live registers)
+      if (p != 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) cv[k] = mm > 0 ? compb[k * THREADS + to] : 0u;
+      }
+    }
     if (p == 0) {
 #pragma unroll
       for (int k = 0; k < N; ++k) {
@@ -362,7 +390,14 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       rr.row = ws_half * rr.row;
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        acc_add<KAHAN>(acc[k], cmp[KAHAN ? k : 0], sym_at(rr, tG[b][2][k], k == N / 2) * v[k]);
+        const cx<float> x = sym_at(rr, tG[b][2][k], k == N / 2) * v[k];
+        if constexpr (KAHAN) {
+          cx<float> c = unpack_bf16x2(cv[k]);
+          acc_add<true>(acc[k], c, x);
+          compb[k * THREADS + to] = pack_bf16x2(c);
+        } else {
+          acc[k] = acc[k] + x;
+        }
         v[k] = fp[k];
       }
     }
@@ -372,6 +407,14 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   packed_acc((A.m - 1) & 3, wj_prev);
 
   cx<float>* out = A.part + ((A.cell0 + cell) * A.nr + j) * KP * NPL * PLANE;
+  cx<float> cfin[KAHAN ? N : 1];
+  if constexpr (KAHAN) {
+    if (p != 0) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) cfin[k] = unpack_bf16x2(compb[k * THREADS + tid]);
+    }
+    __syncthreads();  // every thread holds its compensation before the plane is rewritten
+  }
   if (p != 0) {
     const float cj = (float)crad_j;
     const int q0 = p * p + kyf * kyf;
@@ -381,7 +424,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
       const float g = cj * sp[q0 + kzf * kzf];
       out[(long long)p * PLANE + (long long)l * N + k] = g * acc[k];
       if constexpr (KAHAN) {
-        const cx<float> c = cmp[k];
+        const cx<float> c = cfin[k];
         out[NPL * PLANE + (long long)p * PLANE + (long long)l * N + k] = -g * c;
       }
     }
