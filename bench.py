@@ -14,7 +14,8 @@ measured against the FP64 oracle on a sample of cells.  ``roofline`` is for
 the dominant kernel of the FP32 form (VALU-bound FFT work, algorithmic flops
 per cell-eval = (2 Nr M + Nr + 2) 5 n log2 n + (16 Nr M + 7 Nr) n).
 ``cpu_baseline`` times the FP64 numpy/scipy oracle (8 FFT worker threads) on a
-bounded sample of the same workload.
+bounded sample of the same workload, and the reference's two 2D configurations
+on the oracle (``cpu_baseline.reference_configs``).
 
 Side measurements in the same JSON line: the Kahan-compensated FP32 form, the
 sustained FMA rate of the vector ALU (``valu_fma_peak``), the HBM-bound kernels of
@@ -137,6 +138,51 @@ def cpu_baseline(nb_sample, seed):
     except OSError:
         pass
     return out, ncpu, model
+
+
+def cpu_reference_configs(c1_steps=20, workers=8):
+    """The reference's two 2D configurations on the FP64 oracle (host cores, 8 FFT workers), on
+    the same data as reference_configs: C1 forward Euler steps of 2D 32^2 homogeneous cooling
+    (gain + loss), and one projective outer step of C2 (Nx = 200, two inner steps, each the
+    oracle's minmod TVD increment plus dt Q / eps with the reference's gain-only operator)."""
+    from oracle import collision as OC
+    from oracle import integrators as OI
+    from oracle import quadrature as OQ
+    from oracle import transport as OT
+    sig, ws = OQ.circle_rule(8)
+    out = {}
+    s1 = 2.718491035932
+    op = OC.OracleOperator(2, 32, s1, 32, sig, ws, e=0.8, e0=0.8, gamma=0.0)
+    L = 0.5 * (3 + math.sqrt(2)) * s1
+    dv = 2 * L / 32
+    v = -L + (np.arange(32) + 0.5) * dv
+    vx, vy = np.meshgrid(v, v, indexing="ij")
+    rng = np.random.default_rng(0)
+    cs = [np.array([sg, 0.0]) + rng.normal(0, 0.1, 2) for sg in (1.0, -1.0)]
+    f = sum(np.exp(-((vx - c[0]) ** 2 + (vy - c[1]) ** 2) / 1.0) for c in cs)
+    f /= f.sum() * dv * dv
+    op.collide(f, with_loss=True, workers=workers)
+    t0 = time.perf_counter()
+    for _ in range(c1_steps):
+        OI.euler_step(f, lambda g, h: h * op.collide(g, with_loss=True, workers=workers), 0.01)
+    out["config1"] = {"steps_per_s": c1_steps / (time.perf_counter() - t0), "steps": c1_steps}
+    nx = 200
+    op2 = OC.OracleOperator(2, 32, S3, 32, sig, ws, e=0.9, e0=0.9, gamma=1.0)
+    vx2, vy2 = np.meshgrid(op2.v_center(), op2.v_center(), indexing="ij")
+    x = (np.arange(nx) + 0.5) / nx
+    rho = np.where(x < 0.5, 1.0, 0.125)[:, None, None]
+    T = np.where(x < 0.5, 1.0, 0.8)[:, None, None]
+    f2 = rho / (2 * math.pi * T) * np.exp(-(vx2 ** 2 + vy2 ** 2) / (2 * T))
+    dx = 1.0 / nx
+
+    def df(g, h):
+        return OT.df_advection(g, vx2, h, dx) + h * op2(g, workers=workers) / 1e-3
+
+    t0 = time.perf_counter()
+    OI.proj_euler_step(f2, df, 0.4 * dx / 8, [2], [1e-3])
+    out["config2"] = {"outer_steps_per_s": 1.0 / (time.perf_counter() - t0), "outer_steps": 1}
+    out["workers"] = workers
+    return out
 
 
 def counters_from_profile():
@@ -781,6 +827,8 @@ def main():
                                          "oracle with 8 FFT threads, %.1f s" % (n8, t8),
                                "all_cores": {"value": ra, "cores": ncpu, "cells": na, "seconds": ta},
                                "host": model}
+        if not args.no_ref_configs:
+            out["cpu_baseline"]["reference_configs"] = cpu_reference_configs()
     print(json.dumps(out))
 
 
