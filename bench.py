@@ -14,8 +14,9 @@ measured against the FP64 oracle on a sample of cells.  ``roofline`` is for
 the dominant kernel of the FP32 form (VALU-bound FFT work, algorithmic flops
 per cell-eval = (2 Nr M + Nr + 2) 5 n log2 n + (16 Nr M + 7 Nr) n).
 ``cpu_baseline`` times the FP64 numpy/scipy oracle (8 FFT worker threads) on a
-bounded sample of the same workload, and the reference's two 2D configurations
-on the oracle (``cpu_baseline.reference_configs``).
+bounded sample of the same workload, the reference's two 2D configurations on
+the oracle (``cpu_baseline.reference_configs``) and an estimate of the uneven
+batch's solves/s on it (``cpu_baseline.uneven_batch``).
 
 Side measurements in the same JSON line: the Kahan-compensated FP32 form, the
 sustained FMA rate of the vector ALU (``valu_fma_peak``), the HBM-bound kernels of
@@ -183,6 +184,31 @@ def cpu_reference_configs(c1_steps=20, workers=8):
     out["config2"] = {"outer_steps_per_s": 1.0 / (time.perf_counter() - t0), "outer_steps": 1}
     out["workers"] = workers
     return out
+
+
+def cpu_uneven_estimate(q_sec_3d32, workers=8):
+    """CPU solves/s of the uneven batch's mix, estimated from the FP64 oracle: one timed Q
+    evaluation per 2D 16^2 / 32^2 / 64^2 and 3D 16^3 shape (after one warm-up; the 3D 32^3 time
+    is the cpu_baseline sample's), times the 32 Q evaluations of a TPI2 solve to t = 2 (8 outer
+    steps of 2 x 2 inner steps); the shapes are equally likely, so the mix's rate is
+    5 / (sum of the per-shape solve times)."""
+    from oracle import collision as OC
+    from oracle import quadrature as OQ
+    per = {}
+    for d, n, m in [(2, 16, 8), (2, 32, 8), (2, 64, 8), (3, 16, 38)]:
+        sig, ws = OQ.circle_rule(m) if d == 2 else OQ.lebedev_rule(m)
+        s = 3.0 if n < 32 else S3
+        op = OC.OracleOperator(d, n, s, n, sig, ws, e=0.6, e0=0.6, gamma=1.0)
+        g = np.meshgrid(*([op.v_center()] * d), indexing="ij")
+        f = np.exp(-sum(x ** 2 for x in g) / 2.0)
+        op.collide(f, with_loss=True, workers=workers)
+        t0 = time.perf_counter()
+        op.collide(f, with_loss=True, workers=workers)
+        per["%dd%d" % (d, n)] = time.perf_counter() - t0
+    per["3d32"] = q_sec_3d32
+    solve = {k: 32 * v for k, v in per.items()}
+    return {"solves_per_s_estimate": len(solve) / sum(solve.values()), "q_seconds": per,
+            "q_per_solve": 32, "workers": workers}
 
 
 def counters_from_profile():
@@ -829,6 +855,8 @@ def main():
                                "host": model}
         if not args.no_ref_configs:
             out["cpu_baseline"]["reference_configs"] = cpu_reference_configs()
+        if args.uneven_problems > 0:
+            out["cpu_baseline"]["uneven_batch"] = cpu_uneven_estimate(1.0 / r8)
     print(json.dumps(out))
 
 
