@@ -290,14 +290,15 @@ def serve_run(precision, nreq, rate, seed=5):
     return serving.latency_stats(res), res
 
 
-def serve_sweep_requests(precision, nreq, rate, seed):
+def serve_sweep_requests(precision, nreq, rate, seed, full_time=False):
     """Requests of the swept-rate serving run, shaped like the serving workload: Poisson
     arrivals mixing homogeneous cooling solves distributed as the uneven batch (2D 16^2 /
     32^2 / 64^2, 3D 16^3 / 32^3, TPI2, t_final in {0.5, 0.75, .., 3}) with 1D x 2D jobs
     (Nx in {64, 128, 256}, 32^2 velocity, eps in {1e-2, 1e-3}, Sod-type Maxwellian data,
     hard spheres, e ~ U[0.5, 1]).  The 1D jobs use a projective forward Euler schedule
     that is stable at every Nx (2 inner steps of 6e-5, outer 2.5e-4, inner CFL <= 0.12)
-    and final times 2.5e-4 x U{200..1200}, i.e. 0.05 .. 0.3."""
+    and final times 2.5e-4 x U{200..1200}, i.e. 0.05 .. 0.3 (full_time: 2.5e-4 x U{2000..12000},
+    the workload's U[0.5, 3])."""
     from proj_int_inelastic_amd import serving
     rng = np.random.default_rng(seed)
     arrivals = serving.poisson_arrivals(nreq, rate, rng)
@@ -329,7 +330,8 @@ def serve_sweep_requests(precision, nreq, rate, seed):
             rho = np.where(x < 0.5, 1.0, 0.125)[:, None, None]
             T = np.where(x < 0.5, 1.0, 0.8)[:, None, None]
             f0 = rho / (2 * math.pi * T) * np.exp(-(vx ** 2 + vy ** 2) / (2 * T))
-            reqs.append(serving.Request(rid=i, f0=f0, t_final=2.5e-4 * int(rng.integers(200, 1201)),
+            steps = int(rng.integers(2000, 12001)) if full_time else int(rng.integers(200, 1201))
+            reqs.append(serving.Request(rid=i, f0=f0, t_final=2.5e-4 * steps,
                                         e=float(rng.uniform(0.5, 1.0)), dim=2, n=n, nr=n, m=8, s=s,
                                         gamma=1.0, kn=float(rng.choice([1e-2, 1e-3])),
                                         precision=precision, integrator="PFE", inner_steps=(2,),
@@ -337,14 +339,15 @@ def serve_sweep_requests(precision, nreq, rate, seed):
     return reqs
 
 
-def serve_sweep(rates, nreq, with_fp64):
+def serve_sweep(rates, nreq, with_fp64, full_time=False):
     from proj_int_inelastic_amd import serving
     out = []
     for k, rate in enumerate(rates):
         row = {"rate_per_s": rate, "requests": nreq}
         for prec in (("fp32", "fp64") if with_fp64 else ("fp32",)):
             srv = serving.Server(hbm_budget=32 << 30)
-            res = srv.run(serve_sweep_requests(prec, nreq, rate, seed=100 + k), realtime=True)
+            res = srv.run(serve_sweep_requests(prec, nreq, rate, seed=100 + k, full_time=full_time),
+                          realtime=True)
             row[prec] = serving.latency_stats(res)
             print("serve sweep rate %g %s %s" % (rate, prec, row[prec]), file=sys.stderr, flush=True)
         out.append(row)
@@ -665,6 +668,8 @@ def main():
     ap.add_argument("--serve-sweep", default="",
                     help="comma-separated arrival rates: run only the swept-rate serving workload")
     ap.add_argument("--serve-sweep-requests", type=int, default=32)
+    ap.add_argument("--serve-full-time", action="store_true",
+                    help="1D jobs of the sweep run to t_final in U[0.5, 3] instead of 0.05 .. 0.3")
     ap.add_argument("--cells-2d", type=int, default=4096,
                     help="cells of the 2D 32^2 side measurement (0 = skip)")
     ap.add_argument("--cells-3d16", type=int, default=2048,
@@ -696,8 +701,9 @@ def main():
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
     if args.serve_sweep:
         rates = [float(x) for x in args.serve_sweep.split(",") if x]
-        print(json.dumps({"mode": "serve_sweep", "sweep": serve_sweep(rates, args.serve_sweep_requests,
-                                                                      not args.no_fp64)}))
+        print(json.dumps({"mode": "serve_sweep", "full_time": args.serve_full_time,
+                          "sweep": serve_sweep(rates, args.serve_sweep_requests, not args.no_fp64,
+                                               args.serve_full_time)}))
         return
     if args.mode == "profile2d":
         print(json.dumps({"mode": "profile2d", **run_shape("2d32", args.cells_2d, args.steps, args.warmup,
