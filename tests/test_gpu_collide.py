@@ -528,3 +528,33 @@ def test_single_radial_node_and_angle(pkg, need_gpu, prec, d, N, nr, M):
     q = coll.collide(torch.from_numpy(f).cuda(), with_loss=True).cpu().numpy()
     worst = max(_err(q[c], op.collide(f[c], with_loss=True)) for c in range(2))
     assert worst <= (TOL64 if prec == "fp64" else TOL32), worst
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp32k"])
+@pytest.mark.parametrize("d,N,nr,M,s", [(3, 32, 32, 38, 3.624654714576), (2, 32, 32, 8, 3.624654714576),
+                                        (2, 64, 64, 8, 3.0), (3, 16, 16, 38, 3.0)])
+def test_full_size_scaling_and_shift(pkg, need_gpu, prec, d, N, nr, M, s):
+    """Size-independent properties at the benchmark shapes (full Nr and M), every engine:
+    Q is quadratic, and scaling f by 2 scales every floating-point operation of the path by a
+    power of two, so Q(2f) = 4 Q(f) bit for bit in FP64 (in FP32 the Gaussian tails pass through
+    the subnormal range, where a power-of-two scale changes the rounding, so within 1e-6); a
+    periodic roll of f by one grid point along each velocity axis rolls Q (DFT shift theorem;
+    round-off differs, so within the precision's tolerance)."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    cfg = _cfg(d, N, nr, M, s, 0.8, 0.8, 1.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision=prec)
+    f = torch.from_numpy(_smooth(d, N, sm.L, 3, seed=77)).cuda()
+    e = np.array([0.6, 0.8, 0.95])
+    q = coll.collide(f, with_loss=True, e=e).cpu().numpy()
+    q2 = coll.collide(2.0 * f, with_loss=True, e=e).cpu().numpy()
+    if prec == "fp64":
+        assert np.array_equal(q2, 4.0 * q)
+    else:
+        assert max(_err(q2[c], 4.0 * q[c]) for c in range(3)) <= 1e-6
+    tol = TOL64 if prec == "fp64" else TOL32
+    for ax in range(1, d + 1):
+        qr = coll.collide(torch.roll(f, 1, dims=ax).contiguous(), with_loss=True, e=e).cpu().numpy()
+        err = max(_err(qr[c], np.roll(q[c], 1, axis=ax - 1)) for c in range(3))
+        assert err <= tol, (ax, err)
