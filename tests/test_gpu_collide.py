@@ -558,3 +558,23 @@ def test_full_size_scaling_and_shift(pkg, need_gpu, prec, d, N, nr, M, s):
         qr = coll.collide(torch.roll(f, 1, dims=ax).contiguous(), with_loss=True, e=e).cpu().numpy()
         err = max(_err(qr[c], np.roll(q[c], 1, axis=ax - 1)) for c in range(3))
         assert err <= tol, (ax, err)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp32k", "fp64"])
+def test_full_size_conservation_and_cooling(pkg, need_gpu, prec):
+    """The benchmark shape (3D 32^3, Nr = 32, M = 38, hard spheres, per-cell e < 1) with the
+    conservative projection: mass and momentum of Q vanish to 1e-12 of sum |Q| (the FP64
+    projection, whatever the precision of Q), and the energy moment is negative (inelastic
+    collisions cool)."""
+    import torch
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    cfg = _cfg(3, 32, 32, 38, 3.624654714576, 0.8, 0.8, 1.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm, precision=prec)
+    f = torch.from_numpy(_smooth(3, 32, sm.L, 4, seed=5)).cuda()
+    q = coll.collide(f, with_loss=True, e=np.array([0.5, 0.7, 0.9, 0.99]), project=True)
+    mom = coll.moments(q).cpu().numpy()
+    scale = q.abs().sum(dim=(1, 2, 3)).cpu().numpy() * sm.delta ** 3
+    assert np.all(np.abs(mom[:, 0]) <= 1e-12 * scale), mom[:, 0] / scale
+    assert np.all(np.abs(mom[:, 1:4]) <= 1e-12 * scale[:, None] * sm.L), mom[:, 1:4]
+    assert np.all(mom[:, 4] < 0), mom[:, 4]
