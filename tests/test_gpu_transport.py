@@ -73,6 +73,27 @@ def test_tvd_oracle_ragged(pkg, need_gpu, nx, bc, order):
     assert np.array_equal(df, ref)
 
 
+def test_tvd_large_grid_bit_exact_and_conservative(pkg, need_gpu):
+    """A long grid (Nx = 8192 x 32^2, 8.4M points, several blocks of cells per column):
+    bit-exact against the oracle, and with periodic boundaries the increment sums to zero
+    over x for every velocity to round-off (the flux differences telescope)."""
+    lib = pkg._lib.load()
+    nx, N, L = 8192, 32, 8.0
+    rng = np.random.default_rng(17)
+    v = -L + (np.arange(N) + 0.5) * 2 * L / N
+    v1 = np.repeat(v[:, None], N, axis=1)
+    x = (np.arange(nx) + 0.5) / nx
+    f = (1.0 + 0.5 * np.sin(6 * np.pi * x))[:, None, None] * np.exp(-(v1 ** 2 + v1.T ** 2) / 2)[None]
+    f += 0.01 * rng.random(f.shape)
+    dx = 1.0 / nx
+    dt = 0.9 * dx / L
+    rc, df = _tvd(lib, f, v1, dt / dx, 2, 0, 2, 2)
+    assert rc == 0
+    ref = T.df_advection(f, v1, dt, dx, 2, 2, 2, T.minmod)
+    assert np.array_equal(df, ref)
+    assert np.abs(df.sum(axis=0)).max() <= 1e-12 * np.abs(df).sum(axis=0).max()
+
+
 def test_tvd_errors(pkg, need_gpu):
     lib = pkg._lib.load()
     f = np.zeros((4, 8, 8))
