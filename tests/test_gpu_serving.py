@@ -97,3 +97,22 @@ def test_autotuned_server_matches_default(pkg, need_gpu, tmp_path, monkeypatch, 
     for k in b:
         err = np.abs(a[k].f - b[k].f).max() / np.abs(b[k].f).max()
         assert err <= 1e-12, (k, err)
+
+
+def test_realtime_arrivals_match_offline(pkg, need_gpu):
+    """Requests arriving 5 ms apart on the wall clock (continuous batching admits them into
+    classes that are already stepping) give the results of the offline run, where every
+    request is admitted at once; latency bookkeeping is consistent."""
+    import dataclasses
+    from proj_int_inelastic_amd import serving
+    L = 0.5 * (3 + math.sqrt(2)) * 3.0
+    reqs = [dataclasses.replace(r, arrival=0.005 * i) for i, r in enumerate(_requests(serving.Request, L))]
+    live = serving.Server().run(list(reqs), realtime=True)
+    offline = serving.Server().run(list(reqs), realtime=False)
+    assert sorted(live) == sorted(offline)
+    for k in offline:
+        a, b = live[k], offline[k]
+        assert np.abs(a.f - b.f).max() <= 1e-12 * np.abs(b.f).max(), k
+        assert a.finished >= a.admitted >= a.arrival >= 0.0
+    st = serving.latency_stats(live)
+    assert st["n"] == len(reqs) and 0 < st["p50_s"] <= st["p99_s"]
