@@ -171,6 +171,76 @@ class OracleOperator:
         op.gain = 0.5 * (self.gain + np.conj(self._mirror(self.gain)))
         return op
 
+    def nyquist_term(self, f):
+        """Gain spectrum G_A * fft(f * ifft(E_A fhat)) summed over (j, m), for d = 3.
+
+        E_A, G_A are the anti-Hermitian parts of the translate and gain kernels; they
+        vanish off the three Nyquist planes.  ``symmetrized()`` plus this term is
+        the reference's C2C gain (``Re ifftn(term) / |S|`` added to the symmetrised
+        gain).  Restated through the factorisation the device kernel uses:
+        ifft(E_A fhat) = i beta with beta a sum of three plane waves (-1)^x P1(y,z),
+        (-1)^y P2(x,z), (-1)^z P3(x,y) (2D inverse transforms of the phase-weighted
+        Nyquist planes of fhat) and a corner term; fft(f b) on the plane kx = N/2 is
+        i fft2(sum_x (-1)^x f beta), and likewise on ky = N/2, kz = N/2.
+        Returns the (N, N, N) complex spectrum (zero off the planes), in the units of
+        ``self.gain`` (|S| included).
+        """
+        if self.d != 3:
+            raise ValueError("the Nyquist term is defined here for d = 3")
+        N, L = self.N, self.L
+        p = N // 2
+        f = np.asarray(f, dtype=np.float64)
+        F = np.fft.fftn(f) / f.size
+        kap = spectral_index(N).astype(np.float64)
+        sgn = (-1.0) ** np.arange(N)
+        sx3 = sgn[:, None, None] * sgn[None, :, None] * sgn[None, None, :]
+        out = np.zeros((N, N, N), dtype=np.complex128)
+        notp = np.arange(N) != p
+        for j in range(self.nr):
+            rj, aj = self.r[j], self.exp_fac[j]
+            acc0 = np.zeros((N, N), dtype=np.complex128)
+            acc1 = np.zeros((N, N), dtype=np.complex128)
+            acc2 = np.zeros((N, N), dtype=np.complex128)
+            for m in range(self.M):
+                sg = self.sigma[m]
+                e = [np.exp(-1j * math.pi * rj * sg[a] * kap / L) for a in range(3)]
+                g = [np.exp(1j * aj * rj * sg[a] * kap) for a in range(3)]
+                h = [x.copy() for x in e]
+                for a in range(3):
+                    h[a][p] = e[a][p].real
+                s = [e[a][p].imag for a in range(3)]
+                P1 = (np.fft.ifft2(h[1][:, None] * h[2][None, :] * F[p, :, :]) * N * N).real
+                P2 = (np.fft.ifft2(h[0][:, None] * h[2][None, :] * F[:, p, :]) * N * N).real
+                P3 = (np.fft.ifft2(h[0][:, None] * h[1][None, :] * F[:, :, p]) * N * N).real
+                beta = (s[0] * sgn[:, None, None] * P1[None, :, :]
+                        + s[1] * sgn[None, :, None] * P2[:, None, :]
+                        + s[2] * sgn[None, None, :] * P3[:, :, None]
+                        - s[0] * s[1] * s[2] * F[p, p, p].real * sx3)
+                u = f * beta
+                V1 = np.fft.fft2(np.tensordot(sgn, u, axes=(0, 0)))       # (ky, kz) on kx = p
+                V2 = np.fft.fft2(np.einsum("y,xyz->xz", sgn, u))         # (kx, kz) on ky = p
+                V3 = np.fft.fft2(np.tensordot(u, sgn, axes=(2, 0)))       # (kx, ky) on kz = p
+                # weight w_sigma Im(prod over Nyquist axes of g(p)) * prod over the others of g(k)
+                nyq = [np.where(notp, 1.0, g[a]) for a in range(3)]
+                rest = [np.where(notp, g[a], 1.0) for a in range(3)]
+                w = self.wsigma[m]
+                acc0 += w * (g[0][p] * nyq[1][:, None] * nyq[2][None, :]).imag \
+                    * rest[1][:, None] * rest[2][None, :] * V1
+                acc1 += w * (g[1][p] * nyq[2][None, :] * np.ones((N, 1))).imag \
+                    * rest[0][:, None] * rest[2][None, :] * V2
+                acc2 += w * g[2][p].imag * rest[0][:, None] * rest[1][None, :] * V3
+            crad = self._pre_fac(rj) * self.wr[j]
+            sp = sp_func(3, aj * rj * self.idx_norm)
+            out[p, :, :] += -crad * sp[p, :, :] * acc0
+            t1 = -crad * sp[:, p, :] * acc1
+            t1[p, :] = 0.0                       # kx = p points belong to the first plane
+            out[:, p, :] += t1
+            t2 = -crad * sp[:, :, p] * acc2
+            t2[p, :] = 0.0
+            t2[:, p] = 0.0
+            out[:, :, p] += t2
+        return out
+
     # -- inelastic_collisions.py:208-227 (looped over r for bounded memory)
     def collide_parts(self, f, workers=1):
         """Return (gain_sum_hat_ifft, loss_sum) before the final scaling.

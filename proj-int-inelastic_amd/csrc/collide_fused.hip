@@ -1,14 +1,13 @@
 // Fused FP32 form of Q(f, f) for 3D 32^3 grids (the 3D velocity batch).
 //
 // Same operator as reference src/collision/inelastic_collisions.py:208-227,
-// computed with real-data transforms: h = f * ifft(E fhat) is real, so each
-// 32^3 transform needs only the kx in [0, 16] half spectrum; the kx = 0 and
-// kx = 16 planes (real after the x transform) are packed into one complex
-// plane.  The phase factors E and G are replaced by their Hermitian parts
-// (real part of the product of their Nyquist factors).  This equals the
-// reference's C2C operator exactly when fhat vanishes on the Nyquist planes
-// and differs at first order in the Nyquist content of fhat otherwise (below
-// 2e-7 relative on Gaussian data resolved by the 32^3 grid).
+// computed with real-data transforms: the Hermitian parts E_H, G_H of the phase
+// factors make h = f * ifft(E_H fhat) real, so each 32^3 transform needs only the
+// kx in [0, 16] half spectrum; the kx = 0 and kx = 16 planes (real after the x
+// transform) are packed into one complex plane.  The anti-Hermitian parts E_A, G_A
+// live on the three Nyquist planes only; their term G_A * fft(f * ifft(E_A fhat))
+// is computed exactly by k_f3_nyq (collide_nyq.hip) and added in FP64 before the
+// final transform, so the sum is the reference's C2C operator.
 //
 // Work decomposition: one 512-thread workgroup per (cell, radial node j)
 // loops over the M angles.  The 16 packed kx-planes (32 x 33 padded complex
@@ -42,6 +41,10 @@ int staged_epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void
 // fused C2C engines: FP32 2D and 3D 16^3, FP64 2D 32^2 (collide_fused2.hip)
 int collide_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                    int with_loss, double hb, hipStream_t st, int* handled);
+// exact anti-Hermitian (Nyquist-plane) term of the real-data form (collide_nyq.hip)
+size_t nyq_scratch_bytes(int nr);
+int launch_f3_nyq(Plan* P, const cx<float>* xp, const float* f32, cx<float>* out, cx<double>* acc64,
+                  long long nb, const int* pidx, hipStream_t st);
 
 namespace f3 {
 constexpr int N = 32;
@@ -437,21 +440,27 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   }
 }
 
-// fhat (FP64, full spectrum) -> kx = 0..16 planes in FP32, scaled by 1/n; f -> FP32.
+// fhat (FP64, full spectrum) -> kx = 0..16 planes in FP32, scaled by 1/n; f -> FP32;
+// the three Nyquist planes of fhat / n, [cell][3][32][32]: kx = 16 (ky, kz), ky = 16
+// (kx, kz), kz = 16 (kx, ky), for the anti-Hermitian term.
 __global__ void k_f3_pack(const cx<double>* __restrict__ fhat, const double* __restrict__ f,
-                          cx<float>* __restrict__ fh32, float* __restrict__ f32, long long nb) {
+                          cx<float>* __restrict__ fh32, float* __restrict__ f32,
+                          cx<float>* __restrict__ xp, long long nb) {
   using namespace f3;
   const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long per = NPTS;
   if (gid >= nb * per) return;
   const long long c = gid / per, i = gid % per;
   f32[gid] = (float)f[gid];
-  const int kx = (int)(i / PLANE);
-  if (kx <= N / 2) {
-    const cx<double> v = fhat[gid];
-    const double s = 1.0 / (double)NPTS;
-    fh32[c * 17 * PLANE + i] = cx<float>{(float)(v.re * s), (float)(v.im * s)};
-  }
+  const int kx = (int)(i / PLANE), ky = (int)((i / N) % N), kz = (int)(i % N);
+  const cx<double> v = fhat[gid];
+  const double s = 1.0 / (double)NPTS;
+  const cx<float> vs{(float)(v.re * s), (float)(v.im * s)};
+  if (kx <= N / 2) fh32[c * 17 * PLANE + i] = vs;
+  cx<float>* xc = xp + c * 3 * PLANE;
+  if (kx == N / 2) xc[ky * N + kz] = vs;
+  if (ky == N / 2) xc[PLANE + kx * N + kz] = vs;
+  if (kz == N / 2) xc[2 * PLANE + kx * N + ky] = vs;
 }
 
 // Sum the per-j partials in FP64 and expand the packed half spectrum to the
@@ -507,21 +516,24 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
   const int kp = P->kahan ? 2 : 1;
   const size_t part_bytes = (size_t)P->nr * kp * NPL * PLANE * sizeof(cx<float>);
   const size_t fh32_bytes = (size_t)17 * PLANE * sizeof(cx<float>) + (size_t)NPTS * sizeof(float);
+  const size_t nyq_bytes = nyq_scratch_bytes(P->nr);
   int rc;
   if ((rc = P->fhat.ensure(sub * b64))) return rc;
   if ((rc = P->acc.ensure(sub * b64))) return rc;
   if ((rc = P->work.ensure(2 * sub * b64))) return rc;
-  if ((rc = P->aux.ensure(sub * (part_bytes + fh32_bytes)))) return rc;
+  if ((rc = P->aux.ensure(sub * (part_bytes + fh32_bytes + nyq_bytes)))) return rc;
   cx<float>* part = (cx<float>*)P->aux.ptr;
   cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + sub * part_bytes);
   float* f32 = (float*)((char*)fh32 + sub * 17 * PLANE * sizeof(cx<float>));
+  cx<float>* xp = (cx<float>*)(f32 + sub * NPTS);          // sub x 3 planes
+  cx<float>* nyq_out = xp + sub * 3 * PLANE;               // sub x nr x 3 planes
   for (long long c0 = 0; c0 < batch; c0 += sub) {
     const long long nb = std::min(sub, batch - c0);
     const double* fc = f + c0 * NPTS;
     if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
     const long long tot = nb * NPTS;
     hipLaunchKernelGGL(k_f3_pack, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                       (const cx<double>*)P->fhat.ptr, fc, fh32, f32, nb);
+                       (const cx<double>*)P->fhat.ptr, fc, fh32, f32, xp, nb);
     IB_HIP(hipGetLastError());
     F3Args A{};
     A.fh = fh32;
@@ -554,6 +566,7 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
     hipLaunchKernelGGL(k_f3_reduce, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, st, part,
                        (cx<double>*)P->acc.ptr, nb, P->nr, kp);
     IB_HIP(hipGetLastError());
+    if ((rc = launch_f3_nyq(P, xp, f32, nyq_out, (cx<double>*)P->acc.ptr, nb, A.pidx, st))) return rc;
     if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
                                    with_loss, hb, pidx ? pidx + c0 : nullptr)))
       return rc;

@@ -200,3 +200,37 @@ def test_symmetrized_operator_equals_c2c_for_band_limited_data():
     assert np.abs(sy.collide(fb, with_loss=True) - q).max() < 1e-14 * np.abs(q).max()
     q = op.collide(f, with_loss=True)
     assert np.abs(sy.collide(f, with_loss=True) - q).max() > 1e-6 * np.abs(q).max()
+
+
+@pytest.mark.parametrize("N,nr,M,seed", [(8, 3, 14, 2), (16, 2, 6, 3)])
+def test_nyquist_term_completes_symmetrized(N, nr, M, seed):
+    """The anti-Hermitian (Nyquist-plane) term the FP32 3D form adds to its real-data
+    operator: (i) restated through the plane factorisation it equals the direct sum
+    G_A * fft(f * ifft(E_A fhat)) built from the oracle's own (Nr, M, n) tables, and
+    (ii) the symmetrised operator plus it is the reference's C2C operator, to round-off,
+    on data with O(1) Nyquist content (e != e0, hard spheres, Lebedev weights)."""
+    sig, ws = Qd.lebedev_rule(M)
+    op = C.OracleOperator(3, N, 2.5, nr, sig, ws, e=0.55, e0=0.9, gamma=1.0)
+    rng = np.random.default_rng(seed)
+    f = rng.random((N,) * 3) + 0.3
+    term = op.nyquist_term(f)
+    F = np.fft.fftn(f)
+    direct = np.zeros_like(term)
+    for j in range(nr):
+        for m in range(op.M):
+            E, G = op.exp[j, m], op.gain[j, m]
+            EA = 0.5 * (E - np.conj(op._mirror(E)))
+            GA = 0.5 * (G - np.conj(op._mirror(G)))
+            direct += GA * np.fft.fftn(f * np.fft.ifftn(EA * F))
+    assert np.abs(term - direct).max() <= 1e-13 * np.abs(direct).max()
+    p = N // 2
+    off = np.ones((N,) * 3, dtype=bool)
+    off[p, :, :] = off[:, p, :] = off[:, :, p] = False
+    assert np.abs(term[off]).max() == 0.0
+    # every plane and the corner carry part of the term on this data
+    assert np.abs(term[p, 1, 2]) > 0 and np.abs(term[1, p, 2]) > 0 and np.abs(term[1, 2, p]) > 0
+    qc = op.collide(f)
+    qs = op.symmetrized().collide(f)
+    qfix = qs + (np.fft.ifftn(term) / op.sphr_fac).real
+    assert np.abs(qs - qc).max() > 1e-4 * np.abs(qc).max()  # the term matters here
+    assert np.abs(qfix - qc).max() <= 1e-14 * np.abs(qc).max()
