@@ -21,15 +21,17 @@
 //   -w_sigma Im(prod_{S(k)} g_d(16)) prod_{not S(k)} g_d(k_d) V(k) * r^(g+d-1) w_j sp(a r |k|).
 //
 // Work decomposition: one 512-thread workgroup per (cell, radial node j) loops over
-// the M angles.  Per angle: wave 0 runs the three 2D inverse transforms (P1 + i P2
-// packed in one complex transform, P3 in a second) with one 32-point pencil per lane;
-// all waves form u = sigma_y sigma_z f beta along x pencils (thread (z, y) and
-// (z, y + 16)) and reduce it: over x in registers (R1), over z across the 32 lanes of a
-// half-wave (R3: a lane exchange, then DPP mirrors), over y in registers, across the
-// half-waves (a lane exchange) and across waves through LDS (R2); wave 0 runs the
-// forward transforms (R1 + i R2 packed, R3); all threads weight the plane points and
-// sum over angles in registers.  The per-(cell, j) plane terms go to a buffer that
-// k_f3n_reduce sums over j in FP64 into the full spectrum.
+// the M angles in groups of four.  Wave q of a group runs angle q's 2D inverse
+// transforms (P1 + i P2 packed in one complex transform, P3 in a second) with one
+// 32-point pencil per lane and leaves P1, P2, P3 as transposed real planes; all waves
+// form u = sigma_y sigma_z f beta along x pencils (thread (z, y) and (z, y + 16), f held
+// in registers for the whole kernel) and reduce it, angle by angle: over x in registers
+// (R1), over z across the 32 lanes of a half-wave (R3) and over y across the two
+// pencils, the half-waves and, through LDS, the waves (R2), by recursive halvings in a
+// lane-dependent register order that needs no selects (nyq_xl); wave q runs angle q's
+// forward transforms (R1 + i R2 packed, R3); all threads weight the plane points with
+// separable per-angle tables and sum over angles in registers.  The per-(cell, j) plane
+// terms go to a buffer that k_f3n_reduce sums over j in FP64 into the full spectrum.
 #include "common.h"
 #include "plan.h"
 
@@ -43,6 +45,7 @@ constexpr int THREADS = 512;
 constexpr int PLANE = N * N;
 constexpr int SLOTS = 3 * PLANE;        // plane points per cell, lines counted on each plane
 constexpr int PER = SLOTS / THREADS;    // slots per thread
+constexpr int G = 4;                    // angles per group
 }  // namespace f3n
 
 struct F3NArgs {
@@ -77,47 +80,26 @@ __device__ __forceinline__ float dpp_f(float v, int ctrl_id) {
   const int iv = __builtin_bit_cast(int, v);
   int r;
   switch (ctrl_id) {
-    case 0: r = __builtin_amdgcn_mov_dpp(iv, 0x140, 0xF, 0xF, false); break;  // row_mirror: r <-> 15 - r
-    case 1: r = __builtin_amdgcn_mov_dpp(iv, 0x141, 0xF, 0xF, false); break;  // row_half_mirror: r <-> 7 - r
-    case 2: r = __builtin_amdgcn_mov_dpp(iv, 0x4E, 0xF, 0xF, false); break;   // quad_perm [2,3,0,1]
-    default: r = __builtin_amdgcn_mov_dpp(iv, 0xB1, 0xF, 0xF, false); break;  // quad_perm [1,0,3,2]
+    case 0: r = __builtin_amdgcn_mov_dpp(iv, 0x140, 0xF, 0xF, false); break;  // row_mirror: lane ^ 15
+    case 1: r = __builtin_amdgcn_mov_dpp(iv, 0x141, 0xF, 0xF, false); break;  // row_half_mirror: lane ^ 7
+    case 2: r = __builtin_amdgcn_mov_dpp(iv, 0xB1, 0xF, 0xF, false); break;   // quad_perm [1,0,3,2]: lane ^ 1
+    default: r = __builtin_amdgcn_mov_dpp(iv, 0x4E, 0xF, 0xF, false); break;  // quad_perm [2,3,0,1]: lane ^ 2
   }
   return __builtin_bit_cast(float, r);
 }
 
-// lo(lane) + lo(lane ^ MASK) on lanes whose MASK bit is clear, hi(lane) + hi(lane ^ MASK) on
-// the others (ds_bpermute; this compiler returns only the first of the two registers that
-// v_permlane16/32_swap exchange, so those instructions are not used)
-template <int MASK>
-__device__ __forceinline__ float xchg_add(float lo, float hi, int lane) {
-  asm volatile("" : "+v"(lo), "+v"(hi));  // values, not a lane-dependent load index
-  const bool up = (lane & MASK) != 0;
-  const float keep = up ? hi : lo;
-  const float send = up ? lo : hi;
-  return keep + __shfl_xor(send, MASK);
-}
-
-// Sum of v over the 32 lanes of a half-wave by recursive halving: on return v[0] of
-// lane l holds sum_lanes v[l & 31].  Stage order: lane bit 4 (ds_bpermute), then
-// bits 3, 2, 1, 0 by DPP (mirror partners differ in the stage's bit).
-__device__ __forceinline__ void halfwave_reduce32(float* v, int lane) {
-#pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = xchg_add<16>(v[i], v[i + 16], lane);
-#pragma unroll
-  for (int st = 0; st < 4; ++st) {
-    const int half = 8 >> st;
-    const bool up = (lane >> (3 - st)) & 1;
-#pragma unroll
-    for (int i = 0; i < half; ++i) {
-      // opaque copies: a select between two loads of v would otherwise be folded into one
-      // load at a lane-dependent index, which puts v in scratch memory
-      float lo = v[i], hi = v[i + half];
-      asm volatile("" : "+v"(lo), "+v"(hi));
-      const float keep = up ? hi : lo;
-      const float send = up ? lo : hi;
-      v[i] = keep + dpp_f(send, st);
-    }
-  }
+// Register order of the x pencils.  Register i of lane l holds x = i ^ XL(l), with
+// XL(l) = 4 b2 ^ 12 b3 ^ 16 b4 ^ 16 b5 (b_k the lane's bits), a multiple of 4, so that
+// the reductions over lanes are plain recursive halvings with no lane-dependent
+// selects: a partner lane l ^ m holds, in register i + h, the x that lane l holds in
+// register i whenever XL(m) = h on the bits below 2h.  Over z (the 32 lanes of a
+// half-wave): l ^ 16 (h = 16, ds_bpermute), l ^ 15 (h = 8, DPP row_mirror), l ^ 7
+// (h = 4, DPP row_half_mirror), then a sum over the quad (l ^ 1, l ^ 2) of the four
+// remaining registers.  Over y (the two half-waves): l ^ 32 (h = 16).  Blocks of four
+// registers stay four consecutive x, so the pencils' P values load as 128-bit vectors.
+__device__ __forceinline__ int nyq_xl(int lane) {
+  return (((lane >> 2) & 1) * 4) ^ (((lane >> 3) & 1) * 12) ^ (((lane >> 4) & 1) * 16) ^
+         (((lane >> 5) & 1) * 16);
 }
 
 // Orders a wave's LDS writes before its later reads of other lanes' data.
@@ -133,13 +115,29 @@ __device__ __forceinline__ cx<float> hfac(const cx<float>* row, int k) {
   return k == f3n::H ? cx<float>{v.re, 0.f} : v;
 }
 
+// Per-angle weight tables of the plane points: the weight w_sigma Im(prod_{S(k)} g(16))
+// prod_{not S(k)} g(k_d) is separable on each plane apart from the kz = 16 column:
+//   plane kx = 16:      A0[ky] gz[kz] (kz != 16), B0[ky] (kz = 16)
+//   plane ky = 16:      A1[kx] gz[kz] (kz != 16), B1[kx] (kz = 16)
+//   plane kz = 16:      A2[kx] gy[ky]
+enum NyqTab { W_A0 = 0, W_B0 = 1, W_A1 = 2, W_B1 = 3, W_A2 = 4 };
+
+// Angles are processed in groups of G: the 2D transforms of a group's angles run at
+// once (wave q of waves 0 .. G-1 owns angle q: its 32 rows, then its 32 columns, of
+// both transforms), the N^3 reductions run angle by angle on all waves, and the plane
+// weights of the whole group are applied in one pass.
 __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
   using namespace f3n;
-  __shared__ cx<float> tE[2][3][N];
-  __shared__ cx<float> tG[2][3][N];
-  __shared__ cx<float> Za[N][PAD];
-  __shared__ cx<float> Zb[N][PAD];
-  __shared__ float RP[8][N][N];
+  constexpr int RPS = N + 4;  // padded row of the R2 partials and of P2 (conflict-free 128-bit access)
+  __shared__ cx<float> tE[2][G][3][N];
+  __shared__ cx<float> tG[2][G][3][N];
+  __shared__ cx<float> tW[2][G][5][N];
+  // per angle: complex transform workspaces Za (P1 + i P2, then R1 + i R2) and Zb (P3, R3);
+  // between the transforms the same bytes hold the real planes P1T[z][y], P2T[z][x],
+  // P3T[y][x] (rows of RPS floats: 128-bit accesses of consecutive rows hit distinct banks)
+  // 128-bit LDS accesses below: 16-byte alignment of both arrays
+  __shared__ __align__(16) cx<float> Z[G][2][N][PAD];
+  __shared__ __align__(16) float RP[8][N][RPS];
   const int tid = threadIdx.x;
   const long long id = blockIdx.x;
   const int c8 = (int)(id % 8);
@@ -156,183 +154,245 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
   const float* fc = A.f32 + cell * (long long)PLANE * N;
   const float fppp = X1[H * N + H].re;  // fhat(16, 16, 16) / n (real)
 
-  auto make_tables = [&](int mm, int b) {
-    if (tid < 6 * N) {
-      const int which = tid / (3 * N), ax = (tid % (3 * N)) / N, k = tid % N;
-      const double s = A.sigma[mm * 3 + ax];
+  // phase tables of the group's angles: E and G factors per axis, from t in [0, 6 G N)
+  auto make_tables = [&](int g0, int bb, int t) {
+    const int q = t / (6 * N), r = t % (6 * N);
+    const int which = r / (3 * N), ax = (r % (3 * N)) / N, k = r % N;
+    const int mm = g0 + q;
+    if (mm < A.m) {
+      const double sg = A.sigma[mm * 3 + ax];
       const double kf = (double)kfreq(k, N);
-      double t = which == 0 ? -(rpl_pi * s * kf) : (aj_pi * s * kf);
-      t -= 2.0 * rint(0.5 * t);  // half turns reduced to [-1, 1] in FP64
+      double tt = which == 0 ? -(rpl_pi * sg * kf) : (aj_pi * sg * kf);
+      tt -= 2.0 * rint(0.5 * tt);  // half turns reduced to [-1, 1] in FP64
       float sn, cs;
-      sincospif((float)t, &sn, &cs);
-      (which ? tG : tE)[b][ax][k] = cx<float>{cs, sn};
+      sincospif((float)tt, &sn, &cs);
+      (which ? tG : tE)[bb][q][ax][k] = cx<float>{cs, sn};
+    }
+  };
+  // plane weight tables of the group's angles from the G factors, t in [0, 5 G N)
+  auto make_weights = [&](int g0, int bb, int t) {
+    const int q = t / (5 * N), r = t % (5 * N), tab = r / N, k = r % N;
+    const int mm = g0 + q;
+    if (mm < A.m) {
+      const float ws = (float)A.wsig[mm];
+      const cx<float>* g = &tG[bb][q][0][0];
+      const cx<float> gx = g[H], gy = g[N + H], gz = g[2 * N + H];
+      cx<float> v;
+      if (tab == 0) v = k == H ? cx<float>{ws * (gx * gy).im, 0.f} : (ws * gx.im) * g[N + k];
+      else if (tab == 1) v = k == H ? cx<float>{ws * ((gx * gy) * gz).im, 0.f} : (ws * (gx * gz).im) * g[N + k];
+      else if (tab == 2) v = (ws * gy.im) * g[k];
+      else if (tab == 3) v = (ws * (gy * gz).im) * g[k];
+      else v = (ws * gz.im) * g[k];
+      tW[bb][q][tab][k] = v;
     }
   };
 
   const int lane = tid & 63, wv = tid >> 6;
   const int z = tid & 31, yq = tid >> 5;  // N^3 roles: x pencils (y, z) and (y + 16, z)
+  const int xl = nyq_xl(lane);
   const float sgz = (z & 1) ? -1.f : 1.f, sgy = (yq & 1) ? -1.f : 1.f;
+
+  // f of the two pencils in the register order above, for the whole kernel
+  float f0[N], f1[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int x = i ^ xl;
+    f0[i] = fc[x * PLANE + yq * N + z];
+    f1[i] = fc[x * PLANE + (yq + 16) * N + z];
+  }
 
   cx<float> acc[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) acc[i] = cx<float>{0.f, 0.f};
 
-  make_tables(0, 0);
+  for (int t = tid; t < 6 * G * N; t += THREADS) make_tables(0, 0, t);
   __syncthreads();
-  for (int mm = 0; mm < A.m; ++mm) {
-    const int b = mm & 1;
-    // A (wave 0): P1 + i P2 = ifft2(h_z (h_y X1 + i h_x X2)), P3 = ifft2(h_x h_y X3)
-    if (tid < 64) {
-      const int a = tid & 31;
-      cx<float>(*Z)[PAD] = tid < 32 ? Za : Zb;
+  for (int t = tid; t < 5 * G * N; t += THREADS) make_weights(0, 0, t);
+  for (int g0 = 0, bb = 0; g0 < A.m; g0 += G, bb ^= 1) {
+    const int ng = A.m - g0 < G ? A.m - g0 : G;
+    // A (wave q < ng): P1 + i P2 = ifft2(h_z (h_y X1 + i h_x X2)), P3 = ifft2(h_x h_y X3),
+    // left as the real planes P1T[z][y], P2T[z][x], P3T[y][x]
+    if (wv < ng) {
+      const int q = wv, a = lane & 31, t = lane >> 5;
+      cx<float>(*Zt)[PAD] = Z[q][t];
+      const cx<float>* tEq = &tE[bb][q][0][0];
       cx<float> v[N];
       // rows in blocks of 8 points, so that at most one block's loads are in flight
-      if (tid < 32) {
-        const cx<float> hy = hfac(tE[b][1], a), hx = hfac(tE[b][0], a);
+      if (t == 0) {
+        const cx<float> hy = hfac(tEq + N, a), hx = hfac(tEq, a);
 #pragma unroll
         for (int c0 = 0; c0 < N; c0 += 8) {
 #pragma unroll
           for (int c = c0; c < c0 + 8; ++c) {
             const cx<float> t1 = hy * X1[a * N + c], t2 = hx * X2[a * N + c];
-            v[c] = hfac(tE[b][2], c) * cx<float>{t1.re - t2.im, t1.im + t2.re};
+            v[c] = hfac(tEq + 2 * N, c) * cx<float>{t1.re - t2.im, t1.im + t2.re};
           }
           __builtin_amdgcn_sched_barrier(0);
         }
       } else {
-        const cx<float> hx = hfac(tE[b][0], a);
+        const cx<float> hx = hfac(tEq, a);
 #pragma unroll
         for (int c0 = 0; c0 < N; c0 += 8) {
 #pragma unroll
-          for (int c = c0; c < c0 + 8; ++c) v[c] = (hx * hfac(tE[b][1], c)) * X3[a * N + c];
+          for (int c = c0; c < c0 + 8; ++c) v[c] = (hx * hfac(tEq + N, c)) * X3[a * N + c];
           __builtin_amdgcn_sched_barrier(0);
         }
       }
       fft_reg<N, +1>(v);
 #pragma unroll
-      for (int c = 0; c < N; ++c) Z[a][c] = v[c];
+      for (int c = 0; c < N; ++c) Zt[a][c] = v[c];
       nyq_wave_sync();
 #pragma unroll
-      for (int r = 0; r < N; ++r) v[r] = Z[r][a];
+      for (int r = 0; r < N; ++r) v[r] = Zt[r][a];
       fft_reg<N, +1>(v);
+      nyq_wave_sync();  // every lane of the wave has read its column before the planes overwrite them
+      float* P1T = (float*)&Z[q][0][0][0];
+      float* P2T = P1T + N * RPS;
+      float* P3T = P2T + N * RPS;
+      if (t == 0) {  // column a = z of P1 + i P2
 #pragma unroll
-      for (int r = 0; r < N; ++r) Z[r][a] = v[r];
-    }
-    __syncthreads();
-    // B (all): u = sigma_y sigma_z f beta on the pencils (y, z), (y + 16, z); reductions
-    float r1[2], r3[2];
-    {
-      const float sx = tE[b][0][H].im, sy = tE[b][1][H].im, sz = tE[b][2][H].im;
-      const float C = sx * sy * sz * fppp;
-      const float K2 = sy * sgz, K3 = sz * sgy;
-      float u[2][N];
-      // roles from an opaque copy of tid, so that the pencils' global and LDS addresses
-      // are formed next to their loads each pair instead of occupying registers across
-      // the pair loop
-      int tq = tid;
-      asm volatile("" : "+v"(tq));
-      const int zz = tq & 31, y0 = tq >> 5, y1 = y0 + 16;
-      // sigma_y sigma_z (s_x P1 - C sigma_y sigma_z) of the two pencils
-      const float K1a = sgy * sgz * sx * Za[y0][zz].re - C;
-      const float K1b = sgy * sgz * sx * Za[y1][zz].re - C;
-      const int toff = y0 * N + zz;
-      float s1a = 0.f, s1b = 0.f;
-      // blocks of 8 x, so that the loads of one block are in flight together and no more
-#pragma unroll
-      for (int x0 = 0; x0 < N; x0 += 8) {
-#pragma unroll
-        for (int x = x0; x < x0 + 8; ++x) {
-          const float p2 = Za[x][zz].im;
-          const float ta = fmaf(K2, p2, (x & 1) ? -K1a : K1a);
-          const float tb = fmaf(K2, p2, (x & 1) ? -K1b : K1b);
-          u[0][x] = fc[x * PLANE + toff] * fmaf(K3, Zb[x][y0].re, ta);
-          u[1][x] = fc[x * PLANE + toff + 16 * N] * fmaf(K3, Zb[x][y1].re, tb);
-          s1a = (x & 1) ? s1a - u[0][x] : s1a + u[0][x];
-          s1b = (x & 1) ? s1b - u[1][x] : s1b + u[1][x];
+        for (int r = 0; r < N; r += 4) {
+          *(float4*)&P1T[a * RPS + r] = make_float4(v[r].re, v[r + 1].re, v[r + 2].re, v[r + 3].re);
+          *(float4*)&P2T[a * RPS + r] = make_float4(v[r].im, v[r + 1].im, v[r + 2].im, v[r + 3].im);
         }
-        __builtin_amdgcn_sched_barrier(0);
-      }
-      r1[0] = sgy * sgz * s1a;
-      r1[1] = sgy * sgz * s1b;
-      // R2 over y: the two pencils in registers, the half-waves by permlane32, waves via LDS
-      float t2[H];
+      } else {  // column a = y of P3
 #pragma unroll
-      for (int i = 0; i < H; ++i) t2[i] = xchg_add<32>(u[0][i] + u[1][i], u[0][i + H] + u[1][i + H], lane);
-#pragma unroll
-      for (int i = 0; i < H; ++i) RP[wv][(lane >> 5) * H + i][z] = t2[i];
-      // R3 over z: across the 32 lanes of the half-wave
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        halfwave_reduce32(u[p], lane);
-        r3[p] = sgy * u[p][0];
+        for (int r = 0; r < N; r += 4)
+          *(float4*)&P3T[a * RPS + r] = make_float4(v[r].re, v[r + 1].re, v[r + 2].re, v[r + 3].re);
       }
     }
     __syncthreads();
-    // C (all): Za <- R1 + i R2 ((y, z) and (x, z) share the index), Zb <- R3 (x, y)
+    for (int q = 0; q < ng; ++q) {
+      const float* P1T = (const float*)&Z[q][0][0][0];
+      const float* P2T = P1T + N * RPS;
+      const float* P3T = P2T + N * RPS;
+      // B (all): u = sigma_y sigma_z f beta on the pencils (y, z), (y + 16, z); reductions
+      float r1[2], r3[2][4];
+      {
+        const float sx = tE[bb][q][0][H].im, sy = tE[bb][q][1][H].im, sz = tE[bb][q][2][H].im;
+        const float C = sx * sy * sz * fppp;
+        const float K2 = sy * sgz, K3 = sz * sgy;
+        // roles from an opaque copy of tid, so that the pencils' LDS addresses are formed
+        // next to their loads each angle instead of occupying registers across the loop
+        int tq = tid;
+        asm volatile("" : "+v"(tq));
+        const int zz = tq & 31, y0 = tq >> 5, y1 = y0 + 16, xq = nyq_xl(tq & 63);
+        // sigma_y sigma_z (s_x P1 - C sigma_y sigma_z) of the two pencils
+        const float K1a = sgy * sgz * sx * P1T[zz * RPS + y0] - C;
+        const float K1b = sgy * sgz * sx * P1T[zz * RPS + y1] - C;
+        float u0[N], u1[N];
+        float s1a = 0.f, s1b = 0.f;
 #pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      const int y = yq + 16 * p;
-      float s2 = 0.f;
+        for (int k = 0; k < N / 4; ++k) {
+          const int xb = (4 * k) ^ xq;  // first x of the register block 4k .. 4k + 3
+          const float4 p2 = *(const float4*)&P2T[zz * RPS + xb];
+          const float4 pa = *(const float4*)&P3T[y0 * RPS + xb];
+          const float4 pb = *(const float4*)&P3T[y1 * RPS + xb];
+          const float p2e[4] = {p2.x, p2.y, p2.z, p2.w};
+          const float pae[4] = {pa.x, pa.y, pa.z, pa.w};
+          const float pbe[4] = {pb.x, pb.y, pb.z, pb.w};
 #pragma unroll
-      for (int w = 0; w < 8; ++w) s2 += RP[w][y][z];
-      Za[y][z] = cx<float>{r1[p], sgz * s2};
-      Zb[z][y] = cx<float>{r3[p], 0.f};
+          for (int e = 0; e < 4; ++e) {
+            const int i = 4 * k + e;  // (-1)^x = (-1)^i: xq is even
+            const float ta = fmaf(K2, p2e[e], (e & 1) ? -K1a : K1a);
+            const float tb = fmaf(K2, p2e[e], (e & 1) ? -K1b : K1b);
+            u0[i] = f0[i] * fmaf(K3, pae[e], ta);
+            u1[i] = f1[i] * fmaf(K3, pbe[e], tb);
+            s1a = (e & 1) ? s1a - u0[i] : s1a + u0[i];
+            s1b = (e & 1) ? s1b - u1[i] : s1b + u1[i];
+          }
+        }
+        r1[0] = sgy * sgz * s1a;
+        r1[1] = sgy * sgz * s1b;
+        // R2 over y: the two pencils in registers, the half-waves (l ^ 32), waves via LDS
+        float t2[H];
+#pragma unroll
+        for (int i = 0; i < H; ++i)
+          t2[i] = (u0[i] + u1[i]) + __shfl_xor(u0[i + H] + u1[i + H], 32);
+#pragma unroll
+        for (int k = 0; k < H / 4; ++k)
+          *(float4*)&RP[wv][zz][(4 * k) ^ xq] = make_float4(t2[4 * k], t2[4 * k + 1], t2[4 * k + 2], t2[4 * k + 3]);
+        // R3 over z: recursive halving across the 32 lanes of the half-wave
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          float* u = p ? u1 : u0;
+#pragma unroll
+          for (int i = 0; i < 16; ++i) u[i] += __shfl_xor(u[i + 16], 16);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) u[i] += dpp_f(u[i + 8], 0);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) u[i] += dpp_f(u[i + 4], 1);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) u[i] += dpp_f(u[i], 2);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) u[i] += dpp_f(u[i], 3);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) r3[p][i] = sgy * u[i];  // R3(x = xl + i, y)
+        }
+      }
+      __syncthreads();
+      // C (all): Za <- R1 + i R2 ((y, z) and (x, z) share the index), Zb <- R3 (x, y)
+      {
+        cx<float>(*Za)[PAD] = Z[q][0];
+        cx<float>(*Zb)[PAD] = Z[q][1];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const int y = yq + 16 * p;
+          float s2 = 0.f;
+#pragma unroll
+          for (int w = 0; w < 8; ++w) s2 += RP[w][z][y];
+          Za[y][z] = cx<float>{r1[p], sgz * s2};
+          // lane e of the quad writes x = xl + e (every lane of the quad holds all four)
+          const int e = lane & 3;
+          const float r3e = e == 0 ? r3[p][0] : e == 1 ? r3[p][1] : e == 2 ? r3[p][2] : r3[p][3];
+          Zb[xl + e][y] = cx<float>{r3e, 0.f};
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    // D (wave 0): forward 2D transforms; waves 1-3: the next angle's phase tables
-    if (tid < 64) {
-      const int a = tid & 31;
-      cx<float>(*Z)[PAD] = tid < 32 ? Za : Zb;
+    // D (wave q < ng): forward 2D transforms of angle q; waves G and up: next group's tables
+    if (wv < ng) {
+      const int q = wv, a = lane & 31, t = lane >> 5;
+      cx<float>(*Zt)[PAD] = Z[q][t];
       cx<float> v[N];
 #pragma unroll
-      for (int c = 0; c < N; ++c) v[c] = Z[a][c];
+      for (int c = 0; c < N; ++c) v[c] = Zt[a][c];
       fft_reg<N, -1>(v);
 #pragma unroll
-      for (int c = 0; c < N; ++c) Z[a][c] = v[c];
+      for (int c = 0; c < N; ++c) Zt[a][c] = v[c];
       nyq_wave_sync();
 #pragma unroll
-      for (int r = 0; r < N; ++r) v[r] = Z[r][a];
+      for (int r = 0; r < N; ++r) v[r] = Zt[r][a];
       fft_reg<N, -1>(v);
 #pragma unroll
-      for (int r = 0; r < N; ++r) Z[r][a] = v[r];
-    } else if (mm + 1 < A.m) {
-      int tq = tid - 64;
-      if (tq < 6 * N) {
-        const int which = tq / (3 * N), ax = (tq % (3 * N)) / N, k = tq % N;
-        const double s = A.sigma[(mm + 1) * 3 + ax];
-        const double kf = (double)kfreq(k, N);
-        double t = which == 0 ? -(rpl_pi * s * kf) : (aj_pi * s * kf);
-        t -= 2.0 * rint(0.5 * t);
-        float sn, cs;
-        sincospif((float)t, &sn, &cs);
-        (which ? tG : tE)[b ^ 1][ax][k] = cx<float>{cs, sn};
-      }
+      for (int r = 0; r < N; ++r) Zt[r][a] = v[r];
+    } else if (wv >= G && g0 + G < A.m) {
+      for (int t = tid - G * 64; t < 6 * G * N; t += THREADS - G * 64) make_tables(g0 + G, bb ^ 1, t);
     }
     __syncthreads();
-    // E (all): V on the plane points, weighted by -G_A / (i r^(g+d-1) w_j sp), summed over angles
-    {
-      const float ws = (float)A.wsig[mm];
+    if (g0 + G < A.m)
+      for (int t = tid; t < 5 * G * N; t += THREADS) make_weights(g0 + G, bb ^ 1, t);
+    // E (all): V on the plane points times the separable weights, summed over angles
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int s = tid + i * THREADS;
-        const int plane = s / PLANE, a = (s / N) % N, c = s % N;
-        cx<float> V;
+    for (int i = 0; i < PER; ++i) {
+      const int s = tid + i * THREADS;
+      const int plane = s / PLANE, a = (s / N) % N, c = s % N;
+      for (int q = 0; q < ng; ++q) {
+        cx<float> V, Wt;
         if (plane == 2) {
-          V = Zb[a][c];
+          V = Z[q][1][a][c];
+          Wt = tW[bb][q][W_A2][a] * tG[bb][q][1][c];
         } else {
-          const cx<float> W = Za[a][c], Wm = Za[(N - a) % N][(N - c) % N];
-          V = plane == 0 ? cx<float>{0.5f * (W.re + Wm.re), 0.5f * (W.im - Wm.im)}
-                         : cx<float>{0.5f * (W.im + Wm.im), 0.5f * (Wm.re - W.re)};
+          const cx<float> W = Z[q][0][a][c], Wm = Z[q][0][(N - a) % N][(N - c) % N];
+          if (plane == 0) {
+            V = cx<float>{0.5f * (W.re + Wm.re), 0.5f * (W.im - Wm.im)};
+            Wt = c == H ? tW[bb][q][W_B0][a] : tW[bb][q][W_A0][a] * tG[bb][q][2][c];
+          } else {
+            V = cx<float>{0.5f * (W.im + Wm.im), 0.5f * (Wm.re - W.re)};
+            Wt = c == H ? tW[bb][q][W_B1][a] : tW[bb][q][W_A1][a] * tG[bb][q][2][c];
+          }
         }
-        int k[3];
-        nyq_slot_k(plane, a, c, k);
-        cx<float> nyq{1.f, 0.f}, rest{1.f, 0.f};
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const cx<float> g = tG[b][d][k[d]];
-          if (k[d] == H) nyq = nyq * g;
-          else rest = rest * g;
-        }
-        acc[i] = acc[i] + ((ws * nyq.im) * rest) * V;
+        acc[i] = acc[i] + Wt * V;
       }
     }
     __syncthreads();
