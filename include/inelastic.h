@@ -100,6 +100,18 @@ int ib_plan_get_timing(ib_plan* plan, double* ms_total, long long* launches, lon
 int ib_collide(ib_plan* plan, const double* f, double* q, long long batch,
                const int* param_idx, int with_loss, double heat_bath, void* stream);
 
+/* ib_collide that allocates nothing: the same kernels, with their scratch taken from a
+ * static 128 MiB device array (one per GPU, held until the process ends) instead of the
+ * plan's buffers, which are left untouched.  A batch whose scratch exceeds 128 MiB is
+ * rejected with IB_ERR_ARG before any launch (3D 32^3 FP32: about 7 MiB per cell;
+ * 2D 32^2 FP64: about 0.6 MiB per cell).  Calls on one GPU share the array: calls that
+ * overlap in time on one GPU (other streams, other threads) race.  The kernels assume no
+ * zeroed scratch, so the array's contents from an earlier call do not matter.
+ * No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). */
+int ib_collide_static(ib_plan* plan, const double* f, double* q, long long batch,
+                      const int* param_idx, int with_loss, double heat_bath, void* stream);
+
 /* Grouped form of ib_collide for an uneven batch: nplans plans (any grid shapes,
  * one precision), each with its own cells f[i], q[i], batch[i], param_idx[i]
  * (param_idx may be NULL).  Every stage of the staged engine runs as ONE persistent

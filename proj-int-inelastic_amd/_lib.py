@@ -52,6 +52,7 @@ SIGNATURES = {
     "ib_plan_set_timing": (_I, [_P, _I]),
     "ib_plan_get_timing": (_I, [_P, _c.POINTER(_D), _c.POINTER(_LL), _c.POINTER(_LL)]),
     "ib_collide": (_I, [_P, _P, _P, _LL, _P, _I, _D, _P]),
+    "ib_collide_static": (_I, [_P, _P, _P, _LL, _P, _I, _D, _P]),
     "ib_collide_group": (_I, [_I, _c.POINTER(_P), _c.POINTER(_P), _c.POINTER(_P), _c.POINTER(_LL),
                               _c.POINTER(_P), _I, _D, _P]),
     "ib_laplacian": (_I, [_P, _P, _P, _LL, _P]),

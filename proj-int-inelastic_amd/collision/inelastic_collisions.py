@@ -22,6 +22,9 @@ Extensions (keyword-only, all defaulting to the reference behaviour):
   project     apply the FP64 conservative projection (mass, momentum) to Q.
   autotune    constructor keyword: time the engine's launch-shape candidates
               on the first batch of each size bucket and keep the fastest.
+  static_scratch  ``collide`` keyword: run through ``ib_collide_static``, whose
+              scratch is a static device region (no device allocation; batches
+              whose scratch exceeds it raise ValueError).
 """
 import ctypes
 import math
@@ -182,20 +185,21 @@ class FSInelasticVHSCollision(BaseCollision):
         return out.cpu().numpy()
 
     def collide(self, input_f, with_loss=False, heat_bath=None, e=None, e0=None, project=False,
-                gamma=None):
+                gamma=None, static_scratch=False):
         lib = _lib.load()
+        entry = lib.ib_collide_static if static_scratch else lib.ib_collide
 
         def go(f, out, nb):
-            if self.autotune:
+            if self.autotune and not static_scratch:
                 from .. import autotune as at
                 b = at.bucket(nb)
                 if b not in self._tuned:
                     self._tuned.add(b)  # before timing: tune() calls collide()
                     at.tune(self, f, with_loss=with_loss)
             pidx = self._param_index(nb, e, e0, gamma)
-            _lib.check(lib.ib_collide(self._plan, f.data_ptr(), out.data_ptr(), nb,
-                                      dev.ptr(pidx), int(bool(with_loss)),
-                                      float(heat_bath or 0.0), dev.stream_handle()), "ib_collide")
+            _lib.check(entry(self._plan, f.data_ptr(), out.data_ptr(), nb, dev.ptr(pidx),
+                             int(bool(with_loss)), float(heat_bath or 0.0), dev.stream_handle()),
+                       "ib_collide_static" if static_scratch else "ib_collide")
             if project:
                 _lib.check(lib.ib_project_conservative(out.data_ptr(), nb, self.ndim, self.sm.nv,
                                                        self.sm.L, dev.stream_handle()),

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "common.h"
@@ -14,8 +15,30 @@ namespace ib {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 
+thread_local Arena* g_arena = nullptr;
+
+size_t arena_left() {
+  if (!g_arena) return SIZE_MAX;
+  const size_t at = (g_arena->used + 255) & ~(size_t)255;
+  return at >= g_arena->size ? 0 : g_arena->size - at;
+}
+
 int DevBuf::ensure(size_t want) {
   if (want <= bytes) return IB_OK;
+  if (g_arena) {  // static-scratch call: carve, never allocate
+    const size_t at = (g_arena->used + 255) & ~(size_t)255;
+    if (at > g_arena->size || want > g_arena->size - at) {
+      set_error("ib_collide_static: this batch needs more scratch than the " +
+                std::to_string(g_arena->size >> 20) +
+                " MiB static region holds; pass fewer cells per call (ib_collide lets the "
+                "plan allocate)");
+      return IB_ERR_ARG;
+    }
+    ptr = g_arena->base + at;
+    bytes = want;
+    g_arena->used = at + want;
+    return IB_OK;
+  }
   // growing: work queued earlier on any stream may still read the old buffer
   if (ptr) {
     IB_HIP(hipDeviceSynchronize());
@@ -143,6 +166,30 @@ static int set_params(Plan* p, int n_param, const double* e_r, const double* gam
     rc = launch_phase_tables2(p, st);
     if (rc) return rc;
   }
+  return IB_OK;
+}
+
+// Static scratch of ib_collide_static: one copy per device, held until the process ends.
+// Host code uses the per-device address from hipGetSymbolAddress, never the array's name.
+constexpr size_t kStaticScratchBytes = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[256];
+
+static int static_scratch(void** out) {
+  int dev = 0;
+  IB_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 256) {
+    set_error("device index out of range");
+    return IB_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {  // once per device, with that device current
+    void* p = nullptr;
+    IB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
   return IB_OK;
 }
 
@@ -316,6 +363,43 @@ int ib_collide(ib_plan* p, const double* f, double* q, long long batch, const in
   int rc = collide_fused(p, f, q, batch, param_idx, with_loss, heat_bath, st, &handled);
   if (rc || handled) return rc;
   return collide_plain(p, f, q, batch, param_idx, with_loss, heat_bath, 0, st);
+}
+
+int ib_collide_static(ib_plan* p, const double* f, double* q, long long batch, const int* param_idx,
+                      int with_loss, double heat_bath, void* stream) {
+  if (!p || (!f && batch > 0) || (!q && batch > 0) || batch < 0) {
+    set_error("invalid collide arguments");
+    return IB_ERR_ARG;
+  }
+  if (batch == 0) return IB_OK;
+  void* base = nullptr;
+  int rc = static_scratch(&base);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  p->stream = st;
+  // the call's scratch comes from the static region: the plan's own buffers are set
+  // aside for its duration and restored after, untouched
+  const DevBuf saved[4] = {p->fhat, p->acc, p->work, p->aux};
+  p->fhat = p->acc = p->work = p->aux = DevBuf{};
+  const int timing = p->timing;
+  const size_t budget = p->scratch_budget;
+  p->timing = 0;  // the timing hooks create events
+  p->scratch_budget = std::min(budget, kStaticScratchBytes / 4);
+  Arena ar;
+  ar.base = (char*)base;
+  ar.size = kStaticScratchBytes;
+  g_arena = &ar;
+  int handled = 0;
+  rc = collide_fused(p, f, q, batch, param_idx, with_loss, heat_bath, st, &handled);
+  if (!rc && !handled) rc = collide_plain(p, f, q, batch, param_idx, with_loss, heat_bath, 0, st);
+  g_arena = nullptr;
+  p->fhat = saved[0];
+  p->acc = saved[1];
+  p->work = saved[2];
+  p->aux = saved[3];
+  p->timing = timing;
+  p->scratch_budget = budget;
+  return rc;
 }
 
 int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, double* const* q,

@@ -564,6 +564,8 @@ static int collide_staged(Plan* P, const double* f, double* q, long long batch, 
       if (wg0 < 512 && np >= 4) {
         S = std::min<long long>((1024 + wg0 - 1) / wg0, np / 2);
         S = std::min<long long>(S, std::max<long long>(1, (512LL << 20) / ((long long)nb * arr)));
+        // a static-scratch call: the partials must fit in what the region has left
+        S = std::min<long long>(S, (long long)std::min<size_t>(arena_left() / ((size_t)nb * arr), (size_t)S));
       }
       if (S > 1) {
         IB_TRY(P->aux.ensure((size_t)(nb * S) * arr));

@@ -25,9 +25,19 @@ void set_error(const std::string& msg);
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
-  int ensure(size_t want);  // grow-only; returns IB_OK or IB_ERR_HIP
+  int ensure(size_t want);  // grow-only; returns IB_OK or IB_ERR_HIP (IB_ERR_ARG in an arena)
   void release();
 };
+
+// Caller-provided scratch region of one call (ib_collide_static): while g_arena is set on
+// the calling thread, DevBuf::ensure carves 256-byte-aligned pieces from it instead of
+// allocating, and fails with IB_ERR_ARG when the region is exhausted.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+};
+extern thread_local Arena* g_arena;
+size_t arena_left();  // bytes left in the active arena; SIZE_MAX when none is active
 
 struct Plan {
   // geometry and quadrature (host copies)
