@@ -11,8 +11,12 @@ One step = one Q evaluation of the whole batch (inputs resident in HBM).
 ``value`` is the FP32 form's cell-evals/s; the plain FP64 form is timed on the
 same batch and reported beside it with the speed-up; the FP32 error is
 measured against the FP64 oracle on a sample of cells.  ``roofline`` is for
-the dominant kernel of the FP32 form (VALU-bound FFT work, algorithmic flops
-per cell-eval = (2 Nr M + Nr + 2) 5 n log2 n + (16 Nr M + 7 Nr) n).
+the dominant kernel of the FP32 form, k_fused3 (vector-ALU-bound FFT work, no
+MFMA), on the work it executes: its real-data transforms cover the kx in
+[0, 16] half spectrum, Nr M (5 n log2 n + 8 n) flop per cell-eval; the C2C
+count (2 Nr M + Nr + 2) 5 n log2 n + (16 Nr M + 7 Nr) n is reported beside it
+as ``c2c_equivalent``.  ``counters`` and ``traffic`` are read from the committed
+rocprofv3 summaries of the same kernel (marked ``static``).
 ``cpu_baseline`` times the FP64 numpy/scipy oracle (8 FFT worker threads) on a
 bounded sample of the same workload, the reference's two 2D configurations on
 the oracle (``cpu_baseline.reference_configs``) and an estimate of the uneven
@@ -44,7 +48,15 @@ N, NR, M, D = 32, 32, 38, 3
 
 
 def flops_per_cell(n=N ** D, nr=NR, m=M):
+    """C2C algorithmic flop of one Q(f,f) cell-eval with the loss term."""
     return (2 * nr * m + nr + 2) * 5 * n * math.log2(n) + (16 * nr * m + 7 * nr) * n
+
+
+def flops_per_cell_half(n=N ** D, nr=NR, m=M):
+    """Flop k_fused3 executes per cell-eval: per quadrature pair one inverse and one forward
+    real-data transform (the half spectrum: 5 n log2 n together) and the half-spectrum
+    phase, product and accumulate passes (8 n)."""
+    return nr * m * (5 * n * math.log2(n) + 8 * n)
 
 
 def make_batch(nb, L, seed=0):
@@ -234,60 +246,6 @@ def traffic_from_profile():
         if line.startswith("traffic_bytes_per_cell"):
             return float(line.split()[1])
     return None
-
-
-def serve_run(precision, nreq, rate, seed=5):
-    """Poisson arrivals of homogeneous 2D/3D cooling solves and 1D x 2D jobs.
-
-    Homogeneous: 2D 16^2 / 32^2 (M = 8) and 3D 16^3 (M = 38 Lebedev), Nr = N,
-    Maxwell or hard-sphere kernel, e ~ U[0.2, 1], two-Gaussian data, TPI2 with
-    inner steps (2, 2), dt = (0.01, 0.05, 0.25), t_final in {0.5, 1, 1.5, 2}.
-    1D jobs: Nx in {64, 128}, 32^2 velocity, eps = 1e-2, hard spheres,
-    e ~ U[0.5, 1], Sod-type Maxwellian Riemann data, PFE (2 inner steps of
-    2.5e-4, CFL <= 0.26), dt = 1e-3, t_final in {0.01, 0.02}.  Returns latency
-    statistics and the results.
-    """
-    from proj_int_inelastic_amd import serving
-    rng = np.random.default_rng(seed)
-    arrivals = serving.poisson_arrivals(nreq, rate, rng)
-    reqs = []
-    for i, t_arr in enumerate(arrivals):
-        kind = rng.integers(0, 4)
-        e = float(rng.uniform(0.2, 1.0))
-        gamma = float(rng.integers(0, 2))
-        if kind < 3:
-            d, n = [(2, 16), (2, 32), (3, 16)][kind]
-            m = 8 if d == 2 else 38
-            s = 3.0
-            L = 0.5 * (3 + math.sqrt(2)) * s
-            dv = 2 * L / n
-            v = -L + (np.arange(n) + 0.5) * dv
-            g = np.meshgrid(*([v] * d), indexing="ij")
-            c = rng.normal(0, 0.1, d)
-            f0 = sum(np.exp(-sum((g[a] - sg * (1.0 if a == 0 else 0.0) - c[a]) ** 2 for a in range(d)) / 1.0)
-                     for sg in (1, -1))
-            f0 /= f0.sum() * dv ** d
-            reqs.append(serving.Request(rid=i, f0=f0, t_final=float(rng.choice([0.5, 1.0, 1.5, 2.0])), e=e,
-                                        dim=d, n=n, nr=n, m=m, s=s, gamma=gamma, precision=precision,
-                                        integrator="TPI2", inner_steps=(2, 2), inner_dt=(0.01, 0.05),
-                                        dt=0.25, arrival=float(t_arr)))
-        else:
-            nx, n, s = int(rng.choice([64, 128])), 32, S3
-            L = 0.5 * (3 + math.sqrt(2)) * s
-            v = -L + (np.arange(n) + 0.5) * 2 * L / n
-            vx, vy = np.meshgrid(v, v, indexing="ij")
-            x = (np.arange(nx) + 0.5) / nx
-            rho = np.where(x < 0.5, 1.0, 0.125)[:, None, None]
-            T = np.where(x < 0.5, 1.0, 0.8)[:, None, None]
-            f0 = rho / (2 * math.pi * T) * np.exp(-(vx ** 2 + vy ** 2) / (2 * T))
-            reqs.append(serving.Request(rid=i, f0=f0, t_final=float(rng.choice([0.01, 0.02])),
-                                        e=float(rng.uniform(0.5, 1.0)), dim=2, n=n, nr=n, m=8, s=s,
-                                        gamma=1.0, kn=1e-2, precision=precision, integrator="PFE",
-                                        inner_steps=(2,), inner_dt=(2.5e-4,), dt=1e-3, xlen=1.0,
-                                        arrival=float(t_arr)))
-    srv = serving.Server(hbm_budget=32 << 30)
-    res = srv.run(reqs, realtime=True)
-    return serving.latency_stats(res), res
 
 
 def serve_sweep_requests(precision, nreq, rate, seed, full_time=False):
@@ -584,10 +542,11 @@ SIDE_SHAPES = {
 }
 
 
-def run_shape(name, nb, steps, warmup, with_fp64):
+def run_shape(name, nb, steps, warmup, precs=("fp32", "fp64")):
     """Side measurement on one more shape: nb cells of seeded anisotropic Gaussians
-    (per-axis T ~ U[0.5, 2]), hard spheres, e = 0.9, gain + loss; the fused FP32 form and
-    the plain FP64 form; roofline of the fused kernel from its live HIP-event time."""
+    (per-axis T ~ U[0.5, 2]), hard spheres, e = 0.9, gain + loss, in the listed precisions;
+    roofline of the fused FP32 kernel from its live HIP-event time.  These fused kernels
+    transform the full (complex) spectrum, so the C2C count is the executed work."""
     import torch
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
     d, n, nr, m, s, kname, what = SIDE_SHAPES[name]
@@ -609,7 +568,7 @@ def run_shape(name, nb, steps, warmup, with_fp64):
     res = {"workload": "%s, hard spheres, e=0.9, gain+loss, %d cells" % (what, nb)}
     flop = flops_per_cell(n ** d, nr, m)
     out = {}
-    for prec in (("fp32", "fp64") if with_fp64 else ("fp32",)):
+    for prec in precs:
         c = FSInelasticVHSCollision(cfg, sm, precision=prec)
         for _ in range(max(warmup, 1)):
             c.collide(f_dev, with_loss=True)
@@ -629,17 +588,25 @@ def run_shape(name, nb, steps, warmup, with_fp64):
             kms, kl, ku = c.get_timing()
             kavg = kms / max(kl, 1)
             ach = flop * (ku / max(kl, 1)) / (kavg * 1e-3) / 1e12
-            res["roofline_" + kname] = {"bound": "mfma", "compute_unit": "valu", "achieved": ach,
+            res["roofline_" + kname] = {"bound": "valu", "achieved": ach,
                                         "peak": 157.3, "unit": "TFLOP/s", "frac": ach / 157.3,
                                         "kernel_avg_ms": kavg, "cells_per_launch": ku / max(kl, 1),
                                         "flop_per_cell_alg": flop}
-    if "fp64" in out:
+    if "fp64" in out and "fp32" in out:
         a, b = out["fp32"], out["fp64"]
         axes = tuple(range(1, d + 1))
         res["fp32_over_fp64"] = res["fp32_cell_evals_per_s"] / res["fp64_cell_evals_per_s"]
         res["fp32_vs_fp64_rel_linf_max"] = float(((a - b).abs().amax(dim=axes) /
                                                   b.abs().amax(dim=axes)).max())
     return res
+
+
+_T0 = time.perf_counter()
+
+
+def _progress(msg):
+    """One line on stderr per finished stage."""
+    print("bench: %s done (%.0f s)" % (msg, time.perf_counter() - _T0), file=sys.stderr, flush=True)
 
 
 def main():
@@ -651,20 +618,22 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--check-cells", type=int, default=2)
     ap.add_argument("--cpu-cells", type=int, default=36)
-    ap.add_argument("--mode", choices=["bench", "profile", "profile2d", "profile2d64", "profile3d16",
-                                       "copy"],
+    ap.add_argument("--mode", choices=["bench", "profile", "profile2d", "profile2d32fp64", "profile2d64",
+                                       "profile3d16", "copy"],
                     default="bench",
                     help="profile: FP32 form only on --cells cells (for rocprofv3); "
                          "profile2d: the 2D 32^2 FP32 form only on --cells-2d cells; "
+                         "profile2d32fp64: the 2D 32^2 FP64 form only on --cells-2d cells; "
                          "profile2d64: the 2D 64^2 FP32 form only on --cells-2d64 cells; "
                          "profile3d16: the 3D 16^3 FP32 form only on --cells-3d16 cells; "
                          "copy: a plain 1 GiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-kahan", action="store_true", help="skip the fp32k side measurement")
-    ap.add_argument("--serve-requests", type=int, default=48,
+    ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests of the Poisson-arrival serving run (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=40.0, help="arrival rate (1/s)")
+    ap.add_argument("--serve-rate", type=float, default=10.0, help="arrival rate (1/s)")
+    ap.add_argument("--serve-seed", type=int, default=100, help="seed of the serving request mix")
     ap.add_argument("--serve-sweep", default="",
                     help="comma-separated arrival rates: run only the swept-rate serving workload")
     ap.add_argument("--serve-sweep-requests", type=int, default=32)
@@ -676,8 +645,8 @@ def main():
                     help="cells of the 3D 16^3 side measurement (0 = skip)")
     ap.add_argument("--cells-2d64", type=int, default=1024,
                     help="cells of the 2D 64^2 side measurement (0 = skip)")
-    ap.add_argument("--uneven-problems", type=int, default=256,
-                    help="problems of the uneven homogeneous batch (0 = skip; 4096 = full size)")
+    ap.add_argument("--uneven-problems", type=int, default=4096,
+                    help="problems of the uneven homogeneous batch (0 = skip)")
     ap.add_argument("--no-hbm", action="store_true", help="skip the HBM-bound kernel measurements")
     ap.add_argument("--no-ref-configs", action="store_true",
                     help="skip the step-loop timing of the reference's 2D configurations")
@@ -707,15 +676,19 @@ def main():
         return
     if args.mode == "profile2d":
         print(json.dumps({"mode": "profile2d", **run_shape("2d32", args.cells_2d, args.steps, args.warmup,
-                                                           False)}))
+                                                           ("fp32",))}))
+        return
+    if args.mode == "profile2d32fp64":
+        print(json.dumps({"mode": "profile2d32fp64", **run_shape("2d32", args.cells_2d, args.steps,
+                                                                 args.warmup, ("fp64",))}))
         return
     if args.mode == "profile2d64":
         print(json.dumps({"mode": "profile2d64", **run_shape("2d64", args.cells_2d64, args.steps,
-                                                             args.warmup, False)}))
+                                                             args.warmup, ("fp32",))}))
         return
     if args.mode == "profile3d16":
         print(json.dumps({"mode": "profile3d16", **run_shape("3d16", args.cells_3d16, args.steps,
-                                                             args.warmup, False)}))
+                                                             args.warmup, ("fp32",))}))
         return
 
     cfg = cfg3()
@@ -733,6 +706,7 @@ def main():
     kms, klaunch, kcells = c32.get_timing()
     c32.set_timing(False)
     res = {"fp32_s_per_step": t32}
+    _progress("FP32 3D 32^3 timed steps")
     # PCIe-inclusive rate: numpy in, numpy out (host copies inside the timed region)
     nh = min(256, args.cells)
     fh = np.ascontiguousarray(f[:nh])
@@ -748,6 +722,7 @@ def main():
         res["fp64_s_per_step"] = t64
         d = (q32 - q64).abs().amax(dim=(1, 2, 3)) / q64.abs().amax(dim=(1, 2, 3))
         res["fp32_vs_fp64_gpu_rel_linf_max"] = float(d.max())
+        _progress("FP64 3D 32^3")
     if not args.no_kahan:
         ck = FSInelasticVHSCollision(cfg, sm, precision="fp32k")
         tk, wk, qk = time_path(ck, f_dev, e, 1, 1 if args.warmup else 0)
@@ -756,6 +731,7 @@ def main():
             dk = (qk - q64).abs().amax(dim=(1, 2, 3)) / q64.abs().amax(dim=(1, 2, 3))
             res["fp32k"]["vs_fp64_gpu_rel_linf_max"] = float(dk.max())
         del qk
+        _progress("FP32 Kahan 3D 32^3")
     # error of the FP32 form against the FP64 oracle on a sample of cells
     from oracle import collision as OC
     from oracle import quadrature as OQ
@@ -770,12 +746,15 @@ def main():
     mom = c32.moments(q32)
     fm = c32.moments(f_dev)
     mass_drift = float((mom[:, 0].abs() / fm[:, 0].abs()).max())
+    _progress("oracle check of %d cells" % len(errs))
     cells_s = args.cells / t32
     flop = flops_per_cell()
     # dominant kernel (the fused FP32 kernel), timed live with HIP events on its stream
     kernel_avg_ms = kms / max(klaunch, 1)
     cells_per_launch = kcells / max(klaunch, 1)
-    achieved = flop * cells_per_launch / (kernel_avg_ms * 1e-3) / 1e12
+    flop_exec = flops_per_cell_half()
+    achieved = flop_exec * cells_per_launch / (kernel_avg_ms * 1e-3) / 1e12
+    achieved_c2c = flop * cells_per_launch / (kernel_avg_ms * 1e-3) / 1e12
     peak = 157.3
     tr = traffic_from_profile()
     out = {
@@ -796,15 +775,21 @@ def main():
                    "cells": args.cells, "n": N, "nr": NR, "m": M},
         "fp32_rel_linf_vs_oracle_max": max(errs) if errs else None,
         "fp32_mass_rel_drift_max": mass_drift,
-        # compute-bound: the FP32 compute roofline (157.3 TFLOP/s, the MI355X FP32 figure); the
-        # work runs on the vector ALU (FFTs are no dense contraction, so no MFMA instruction)
-        "roofline": {"bound": "mfma", "compute_unit": "valu", "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s",
+        # compute-bound on the vector ALU (FFTs are no dense contraction: no MFMA instruction);
+        # peak = the published MI355X FP32 figure, which counts packed dual-issue FP32 (kept off
+        # in this build); frac_of_attainable compares with the measured non-packed FMA rate
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
                      "traffic": tr * cells_per_launch if tr is not None else None,
+                     "traffic_source": "static: profiles/fused3_pmc_traffic.txt (rocprofv3 --pmc "
+                                       "FETCH_SIZE, WRITE_SIZE of k_fused3, copy-calibrated)",
                      "kernel": "k_fused3", "kernel_avg_ms": kernel_avg_ms,
                      "cells_per_launch": cells_per_launch,
-                     "flop_per_cell_alg": flop, "kernel_share_of_step": kms / (t32 * 1e3 * args.steps)},
+                     "flop_per_cell_executed": flop_exec,
+                     "c2c_equivalent": {"flop_per_cell": flop, "achieved": achieved_c2c,
+                                        "frac": achieved_c2c / peak,
+                                        "note": "the C2C algorithmic count charged to the same time"},
+                     "kernel_share_of_step": kms / (t32 * 1e3 * args.steps)},
     }
     out["pcie_inclusive_cell_evals_per_s"] = res["pcie_inclusive_cell_evals_per_s"]
     if "fp32k" in res:
@@ -813,43 +798,61 @@ def main():
         out["fp64_plain_cell_evals_per_s"] = args.cells / res["fp64_s_per_step"]
         out["fp32_over_fp64"] = res["fp64_s_per_step"] / t32
         out["fp32_vs_fp64_gpu_rel_linf_max"] = res["fp32_vs_fp64_gpu_rel_linf_max"]
-    out["roofline"]["counters"] = counters_from_profile()
+    ctr = counters_from_profile()
+    out["roofline"]["counters"] = dict(ctr, source="static: profiles/fused3_sq_summary.txt "
+                                                   "(rocprofv3 --pmc SQ counters of k_fused3)") if ctr else None
     pk = valu_peak()
     out["roofline"]["measured_valu_fma_peak_tflops"] = pk["fp32_tflops"]
-    out["roofline"]["frac_of_measured_valu_peak"] = achieved / pk["fp32_tflops"]
+    out["roofline"]["frac_of_attainable"] = achieved / pk["fp32_tflops"]
     out["valu_fma_peak"] = dict(pk, note="FMA-loop probe, no packed FP32 (the library's build flags)")
     if not args.no_hbm:
         out["hbm_kernels"] = hbm_kernels()
+        _progress("HBM-bound kernels")
+    precs = ("fp32",) if args.no_fp64 else ("fp32", "fp64")
     if args.cells_2d > 0:
-        out["shape_2d_32"] = run_shape("2d32", args.cells_2d, args.steps, args.warmup, not args.no_fp64)
+        out["shape_2d_32"] = run_shape("2d32", args.cells_2d, args.steps, args.warmup, precs)
+        _progress("2D 32^2 shape")
     if args.cells_3d16 > 0:
-        out["shape_3d_16"] = run_shape("3d16", args.cells_3d16, args.steps, args.warmup, not args.no_fp64)
+        out["shape_3d_16"] = run_shape("3d16", args.cells_3d16, args.steps, args.warmup, precs)
+        _progress("3D 16^3 shape")
     if args.cells_2d64 > 0:
-        out["shape_2d_64"] = run_shape("2d64", args.cells_2d64, args.steps, args.warmup, not args.no_fp64)
+        out["shape_2d_64"] = run_shape("2d64", args.cells_2d64, args.steps, args.warmup, precs)
+        _progress("2D 64^2 shape")
     if not args.no_ref_configs:
         out["reference_configs_fp64"] = reference_configs()
+        _progress("reference configurations")
     if args.uneven_problems > 0:
         u32, r32 = uneven_run("fp32", args.uneven_problems, autotune=not args.no_uneven_autotune)
+        _progress("uneven batch FP32")
         out["uneven_batch"] = {"workload": "homogeneous cooling, 2D 16^2/32^2/64^2 + 3D 16^3/32^3, "
                                            "TPI2 to t=2, grouped launches", "fp32": u32}
         if not args.no_fp64:
             u64, r64 = uneven_run("fp64", args.uneven_problems)
             out["uneven_batch"]["fp64"] = u64
+            _progress("uneven batch FP64")
             out["uneven_batch"]["fp32_over_fp64"] = u32["solves_per_s"] / u64["solves_per_s"]
             errs = [np.abs(r32[k].f - r64[k].f).max() / np.abs(r64[k].f).max() for k in r64]
             out["uneven_batch"]["fp32_vs_fp64_final_f_rel_linf_max"] = float(max(errs))
     if args.serve_requests > 0:
-        st32, sr32 = serve_run("fp32", args.serve_requests, args.serve_rate)
-        out["serving"] = {"fp32": st32,
-                          "workload": "Poisson arrivals (rate %.0f/s) of %d solves: homogeneous 2D 16^2/32^2, "
-                                      "3D 16^3 TPI2 cooling + 1D x 2D PFE jobs" % (args.serve_rate,
-                                                                                   args.serve_requests)}
-        if not args.no_fp64:
-            st64, sr64 = serve_run("fp64", args.serve_requests, args.serve_rate)
-            out["serving"]["fp64"] = st64
-            errs = [np.abs(sr32[k].f - sr64[k].f).max() / np.abs(sr64[k].f).max() for k in sr64]
+        from proj_int_inelastic_amd import serving
+        out["serving"] = {"workload": "Poisson arrivals (rate %g/s) of %d solves: 3/4 homogeneous TPI2 "
+                                      "cooling (2D 16^2/32^2/64^2, 3D 16^3/32^3, t_final in {0.5 .. 3}), "
+                                      "1/4 1D x 2D 32^2 PFE jobs (Nx in {64, 128, 256}, eps in "
+                                      "{1e-2, 1e-3}, t_final = 2.5e-4 x U{2000..12000})"
+                                      % (args.serve_rate, args.serve_requests)}
+        sres = {}
+        for prec in (("fp32", "fp64") if not args.no_fp64 else ("fp32",)):
+            srv = serving.Server(hbm_budget=32 << 30)
+            sres[prec] = srv.run(serve_sweep_requests(prec, args.serve_requests, args.serve_rate,
+                                                      seed=args.serve_seed, full_time=True),
+                                 realtime=True)
+            out["serving"][prec] = serving.latency_stats(sres[prec])
+            print("serving %s %s" % (prec, out["serving"][prec]), file=sys.stderr, flush=True)
+        if "fp64" in sres:
+            errs = [np.abs(sres["fp32"][k].f - sres["fp64"][k].f).max() / np.abs(sres["fp64"][k].f).max()
+                    for k in sres["fp64"]]
             out["serving"]["fp32_vs_fp64_final_f_rel_linf_max"] = float(max(errs))
-        out["p99_solve_latency_s"] = st32["p99_s"]
+        out["p99_solve_latency_s"] = out["serving"]["fp32"]["p99_s"]
     if not args.no_cpu:
         rates, ncpu, model = cpu_baseline(args.cpu_cells, seed=1)
         r8, n8, t8 = rates[8]
