@@ -118,8 +118,10 @@ int ib_collide_static(ib_plan* plan, const double* f, double* q, long long batch
  * kernel launch over the work of all plans (up to 8 plans per launch); plans
  * whose shape has a fused kernel (FP32 3D 32^3 and 16^3, 2D 16^2, 32^2 and 64^2;
  * FP64 2D 32^2) run it, and a plan whose batch is too large for chunks of 32
- * quadrature pairs within its scratch budget runs alone in sub-batches.  New: the
- * reference evaluates one operator per call (src/collision/base.py:32-58). */
+ * quadrature pairs within its scratch budget runs alone in sub-batches.  A plan may
+ * appear only once (its scratch and parameter-set tables serve one segment): a
+ * repeated plan is IB_ERR_ARG before any work is queued.  New: the reference
+ * evaluates one operator per call (src/collision/base.py:32-58). */
 int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, double* const* q,
                      const long long* batch, const int* const* param_idx, int with_loss,
                      double heat_bath, void* stream);
@@ -184,6 +186,14 @@ int ib_tvd_axis(const double* f, double* df, long long outer, int n, long long i
 /* y <- y + a x over n float64 values.  Replaces the `state.f += M * df` updates of
  * src/boltzmann.py:84,126-128,137-140,153-158. */
 int ib_axpy(double* y, const double* x, double a, long long n, void* stream);
+
+/* y <- y + a (b (c x)) over n float64 values, the products taken in that order.  Replaces the
+ * chained increments of the telescopic steps, src/boltzmann.py:138-140 (df1 = M0 * df0;
+ * state.f += df1; state.f += M1 * df1) and :153-158 (df2 = M1 * df1; state.f += M2 * df2):
+ * the caller keeps df0 and passes the multipliers, so df1 and df2 are never stored, and the
+ * result rounds as those numpy products do. */
+int ib_axpy_chain(double* y, const double* x, double a, double b, double c, long long n,
+                  void* stream);
 
 /* out <- adv + (dt * q) / kn (adv may be NULL for 0).  Replaces src/boltzmann.py:160-164 (df). */
 int ib_df_combine(const double* adv, const double* q, double* out, double dt, double kn,

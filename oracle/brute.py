@@ -9,14 +9,46 @@ unnormalised, inverse 1/n):
   Q+hat(k) = 1/|S| sum_{j,m} G_jm(k) (1/n) sum_l E_jm(l) fhat(l) fhat(k - l)
   Q-(x)    = f(x) sum_j l_j (1/n) sum_k S_j(k) fhat(k) exp(2 pi i k.x / N)
 
-with G, E, S, l from the closed forms of :57-114 evaluated here directly
-(not taken from the operator under test).  k - l is taken modulo N per axis.
+with G, E, S, l from the closed forms of :57-114 evaluated here directly:
+nothing is taken from the operator under test.  Its kernel tables are built
+from other formulas of the same functions, so a slip in the oracle's tables
+cannot cancel: the restitution profile as the logistic blend
+e + (e0 - e) / (1 + exp(-2 (r - R/2))) (equal to the tanh form of :77-79),
+J0 from its integral (1/pi) int_0^pi cos(x sin t) dt by the trapezoidal rule
+(exact to round-off for this periodic integrand at 64 nodes and |x| < 40),
+sin(x)/x with its x = 0 limit, |S^1| = 2 pi, |S^2| = 4 pi, and the frequency
+of FFT index q as q - N [q >= N/2].  k - l is taken modulo N per axis.
 """
 import math
 
 import numpy as np
 
-from .collision import e_profile, sp_func, sphere_area, spectral_index
+
+def _restitution(r, R, e, e0):
+    return e + (e0 - e) / (1.0 + np.exp(-2.0 * (np.asarray(r, dtype=np.float64) - 0.5 * R)))
+
+
+def _j0(x):
+    t = (np.arange(64) + 0.5) * (math.pi / 64)   # midpoint rule on [0, pi] = trapezoid of the even extension
+    x = np.asarray(x, dtype=np.float64)
+    return np.cos(x[..., None] * np.sin(t)).mean(axis=-1)
+
+
+def _spherical_mean(d, x):
+    x = np.asarray(x, dtype=np.float64)
+    if d == 2:
+        return _j0(x)
+    safe = np.where(x == 0, 1.0, x)
+    return np.where(x == 0, 1.0, np.sin(safe) / safe)
+
+
+def _area(d):
+    return {2: 2 * math.pi, 3: 4 * math.pi}[d]
+
+
+def _freq(N):
+    q = np.arange(N)
+    return np.where(q >= N // 2, q - N, q)
 
 
 def _dft_mats(N):
@@ -39,12 +71,12 @@ def brute_collide(f, d, N, L, R, r, wr, sigma, wsigma, e, e0, gamma):
     """Return (Q+, Q-) real parts for one cell f of shape N^d."""
     F, Fi = _dft_mats(N)
     fh = _apply_axes(F, f.astype(np.complex128), d)
-    kint = spectral_index(N).astype(np.float64)
+    kint = _freq(N).astype(np.float64)
     grids = np.meshgrid(*([kint] * d), indexing="ij")
     knorm = np.sqrt(sum(g * g for g in grids))
-    S = sphere_area(d)
+    S = _area(d)
     n = N ** d
-    e_r = e_profile(np.asarray(r), R, e, e0)
+    e_r = _restitution(r, R, e, e0)
     a = math.pi * (1 + e_r) / (4 * L)
     qh = np.zeros((N,) * d, dtype=np.complex128)
     for j in range(len(r)):
@@ -52,7 +84,7 @@ def brute_collide(f, d, N, L, R, r, wr, sigma, wsigma, e, e0, gamma):
             sk = sum(sigma[m, ax] * grids[ax] for ax in range(d))
             E = np.exp(-1j * math.pi * r[j] * sk / L)
             G = (S * r[j] ** (gamma + d - 1) * wr[j] * wsigma[m]
-                 * np.exp(1j * a[j] * r[j] * sk) * sp_func(d, a[j] * r[j] * knorm))
+                 * np.exp(1j * a[j] * r[j] * sk) * _spherical_mean(d, a[j] * r[j] * knorm))
             A = E * fh
             conv = np.zeros_like(qh)
             # conv(k) = sum_l A(l) fh(k - l): shift fh by l for every l
@@ -62,6 +94,6 @@ def brute_collide(f, d, N, L, R, r, wr, sigma, wsigma, e, e0, gamma):
     gain = _apply_axes(Fi, qh, d) / S
     lsym = np.zeros((N,) * d)
     for j in range(len(r)):
-        lsym += S * r[j] ** (gamma + d - 1) * wr[j] * sp_func(d, math.pi * r[j] * knorm / L)
+        lsym += S * r[j] ** (gamma + d - 1) * wr[j] * _spherical_mean(d, math.pi * r[j] * knorm / L)
     loss = f * _apply_axes(Fi, lsym * fh, d)
     return gain.real, loss.real

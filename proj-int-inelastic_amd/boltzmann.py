@@ -148,40 +148,41 @@ class BoltzmannSolver(Solver):
             self._axpy(state.f, df, 1.0)
         self._axpy(state.f, df, M)
 
-    # -- boltzmann.py:130-140 (df1 = M0 df0 as written there)
+    @staticmethod
+    def _axpy_chain(y, x, a, b=1.0, c=1.0):
+        """y += a * (b * (c * x)) in one HIP kernel, rounding as those products do."""
+        _lib.check(_lib.load().ib_axpy_chain(y.data_ptr(), x.data_ptr(), float(a), float(b), float(c),
+                                             y.numel(), dev.stream_handle()), "ib_axpy_chain")
+
+    # -- boltzmann.py:130-140 (df1 = M0 df0 as written there).  df1 is never stored: the last
+    # df0 and the multipliers go to one chained update per use
     def tel_proj_step2(self, solution, take_one_step, tstart, tend):
         state = solution.state
         M1 = self.dt / self.inner_dt[1] - self.inner_steps[1]
-        df1 = None
+        df0 = M0 = None
         for _ in range(self.inner_steps[1]):
             M0 = self.inner_dt[1] / self.inner_dt[0] - self.inner_steps[0]
-            df0 = None
             for _ in range(self.inner_steps[0]):
                 df0 = self.df(state, self.inner_dt[0])
                 self._axpy(state.f, df0, 1.0)
-            df1 = M0 * df0
-            self._axpy(state.f, df1, 1.0)
-        self._axpy(state.f, df1, M1)
+            self._axpy_chain(state.f, df0, M0)             # df1 = M0 df0; f += df1
+        self._axpy_chain(state.f, df0, M1, M0)             # f += M1 df1
 
     # -- boltzmann.py:142-158
     def tel_proj_step3(self, solution, take_one_step, tstart, tend):
         state = solution.state
         M2 = self.dt / self.inner_dt[2] - self.inner_steps[2]
-        df2 = None
+        df0 = M0 = M1 = None
         for _ in range(self.inner_steps[2]):
             M1 = self.inner_dt[2] / self.inner_dt[1] - self.inner_steps[1]
-            df1 = None
             for _ in range(self.inner_steps[1]):
                 M0 = self.inner_dt[1] / self.inner_dt[0] - self.inner_steps[0]
-                df0 = None
                 for _ in range(self.inner_steps[0]):
                     df0 = self.df(state, self.inner_dt[0])
                     self._axpy(state.f, df0, 1.0)
-                df1 = M0 * df0
-                self._axpy(state.f, df1, 1.0)
-            df2 = M1 * df1
-            self._axpy(state.f, df2, 1.0)
-        self._axpy(state.f, df2, M2)
+                self._axpy_chain(state.f, df0, M0)         # df1 = M0 df0; f += df1
+            self._axpy_chain(state.f, df0, 1.0, M1, M0)    # df2 = M1 df1; f += df2
+        self._axpy_chain(state.f, df0, M2, M1, M0)         # f += M2 df2
 
     # -- boltzmann.py:160-164
     def df(self, state, dt):

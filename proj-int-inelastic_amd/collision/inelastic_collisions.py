@@ -241,6 +241,9 @@ def collide_group(colls, fs, with_loss=False, heat_bath=None, es=None, gammas=No
     lib = _lib.load()
     t = dev.torch()
     n = len(colls)
+    if len({id(c) for c in colls}) != n:
+        # one operator's parameter-set tables and scratch serve one segment of the launch
+        raise ValueError("an operator appears twice in collide_group; concatenate its cells instead")
     outs, keep, pidx = [], [], []
     nb = (ctypes.c_longlong * n)()
     for i, (c, f) in enumerate(zip(colls, fs)):

@@ -97,6 +97,8 @@ int launch_tvd1d(const double* f, double* df, long long outer, int nx, long long
                  const double* v1, long long vmod, double dtdx, int order, int limiter, int bc_lo,
                  int bc_hi, int accumulate, hipStream_t st);
 int launch_axpy(double* y, const double* x, double a, long long n, hipStream_t st);
+int launch_axpy_chain(double* y, const double* x, double a, double b, double c, long long n,
+                      hipStream_t st);
 int launch_df_combine(const double* adv, const double* q, double* out, double dt, double kn,
                       long long n, hipStream_t st);
 int launch_moments(const double* f, double* out, long long batch, int dim, int n, double L,
@@ -426,6 +428,16 @@ int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, 
       set_error("all plans of a grouped launch must have the same precision");
       return IB_ERR_ARG;
     }
+    // a plan's scratch and parameter-set tables serve one segment: the same plan twice
+    // would share them inside one launch
+    for (int j = 0; j < i; ++j)
+      if (plans[j] == p) {
+        set_error("a plan appears twice in one grouped launch; concatenate its cells instead");
+        return IB_ERR_ARG;
+      }
+  }
+  for (int i = 0; i < nplans; ++i) {
+    ib_plan* p = plans[i];
     if (batch[i] == 0) continue;
     p->stream = st;
     const int* pi = param_idx ? param_idx[i] : nullptr;
@@ -567,6 +579,16 @@ int ib_axpy(double* y, const double* x, double a, long long n, void* stream) {
   }
   if (n == 0) return IB_OK;
   return launch_axpy(y, x, a, n, (hipStream_t)stream);
+}
+
+int ib_axpy_chain(double* y, const double* x, double a, double b, double c, long long n,
+                  void* stream) {
+  if (n < 0 || (n > 0 && (!y || !x))) {
+    set_error("invalid axpy_chain arguments");
+    return IB_ERR_ARG;
+  }
+  if (n == 0) return IB_OK;
+  return launch_axpy_chain(y, x, a, b, c, n, (hipStream_t)stream);
 }
 
 int ib_df_combine(const double* adv, const double* q, double* out, double dt, double kn,

@@ -7,7 +7,7 @@
 //    (src/boltzmann.py:177-205), fused into one stencil: thread (i, v) reads
 //    the five cells i-2..i+2 at velocity v (coalesced over v).  The same kernel
 //    runs along either space axis of a 2D-space grid (dimension-split sum).
-//  * k_axpy, k_df_combine: the in-place updates of src/boltzmann.py:84-164.
+//  * k_axpy, k_axpy_chain, k_df_combine: the in-place updates of src/boltzmann.py:84-164.
 //  * k_moments, k_project: FP64 moments (src/state.py:31-52, any d) and the
 //    new conservative L2 projection of Q.
 //  * k_upwind: the flux-form second-order upwind scheme with the van Leer
@@ -118,6 +118,16 @@ __global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, dou
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     y[i] = y[i] + a * x[i];
+}
+
+// y + a (b (c x)), multiplied in that order: the chained increments df1 = M0 df0,
+// df2 = M1 df1 of the telescopic steps without materialising df1 or df2, rounding as
+// the sequence of scalar-array products does (a factor of 1 is exact)
+__global__ void k_axpy_chain(double* __restrict__ y, const double* __restrict__ x, double a, double b,
+                             double c, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] = y[i] + a * (b * (c * x[i]));
 }
 
 __global__ void k_df_combine(const double* __restrict__ adv, const double* __restrict__ q,
@@ -335,6 +345,13 @@ static unsigned grid_for(long long n) {
 
 int launch_axpy(double* y, const double* x, double a, long long n, hipStream_t st) {
   hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(256), 0, st, y, x, a, n);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+int launch_axpy_chain(double* y, const double* x, double a, double b, double c, long long n,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_chain, dim3(grid_for(n)), dim3(256), 0, st, y, x, a, b, c, n);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }

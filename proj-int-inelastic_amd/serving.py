@@ -14,8 +14,9 @@ waits in the queue (back-pressure) instead of failing.
 
 Step of one tick, for a class with integrator "TPI2" and parameters
 (inner_steps, inner_dt, dt), is the reference's tel_proj_step2
-(src/boltzmann.py:130-140) applied to the stacked states; "PFE" is
-proj_euler_step (:120-128); "Euler" is euler_step (:82-84).  df of each
+(src/boltzmann.py:130-140) applied to the stacked states; "TPI3" is
+tel_proj_step3 (:142-158), "PFE" proj_euler_step (:120-128), "Euler"
+euler_step (:82-84).  df of each
 problem is its TVD transport increment (1D jobs) plus dt Q(f)/kn.
 """
 import math
@@ -208,6 +209,16 @@ def _axpy(y, x, a):
                                    dev.stream_handle()), "ib_axpy")
 
 
+def _axpy_chain(y, x, a, b=1.0, c=1.0):
+    """y += a * (b * (c * x)): the telescopic increments df1 = M0 df0, df2 = M1 df1 without
+    storing them."""
+    _lib.check(_lib.load().ib_axpy_chain(y.data_ptr(), x.data_ptr(), float(a), float(b), float(c),
+                                         y.numel(), dev.stream_handle()), "ib_axpy_chain")
+
+
+INTEGRATORS = ("Euler", "PFE", "TPI2", "TPI3")
+
+
 class _Group:
     """Classes (grid shapes) sharing one integrator schedule.
 
@@ -245,22 +256,39 @@ class _Group:
             for f, d in zip(fs, ds):
                 _axpy(f, d, M)
         elif p.integrator == "TPI2":
-            M1 = p.dt / p.inner_dt[1] - p.inner_steps[1]
-            d1 = None
-            for _ in range(p.inner_steps[1]):
-                M0 = p.inner_dt[1] / p.inner_dt[0] - p.inner_steps[0]
-                d0 = None
-                for _ in range(p.inner_steps[0]):
-                    d0 = self.df(p.inner_dt[0])
+            K, h = p.inner_steps, p.inner_dt
+            M0, M1 = h[1] / h[0] - K[0], p.dt / h[1] - K[1]
+            d0 = None
+            for _ in range(K[1]):
+                d0 = self._inner_block(fs, K[0], h[0])
+                for f, d in zip(fs, d0):
+                    _axpy_chain(f, d, M0)                  # df1 = M0 df0; f += df1
+            for f, d in zip(fs, d0):
+                _axpy_chain(f, d, M1, M0)                  # f += M1 df1
+        elif p.integrator == "TPI3":
+            K, h = p.inner_steps, p.inner_dt
+            M0, M1, M2 = h[1] / h[0] - K[0], h[2] / h[1] - K[1], p.dt / h[2] - K[2]
+            d0 = None
+            for _ in range(K[2]):
+                for _ in range(K[1]):
+                    d0 = self._inner_block(fs, K[0], h[0])
                     for f, d in zip(fs, d0):
-                        _axpy(f, d, 1.0)
-                d1 = [M0 * d for d in d0]
-                for f, d in zip(fs, d1):
-                    _axpy(f, d, 1.0)
-            for f, d in zip(fs, d1):
-                _axpy(f, d, M1)
+                        _axpy_chain(f, d, M0)              # df1 = M0 df0; f += df1
+                for f, d in zip(fs, d0):
+                    _axpy_chain(f, d, 1.0, M1, M0)         # df2 = M1 df1; f += df2
+            for f, d in zip(fs, d0):
+                _axpy_chain(f, d, M2, M1, M0)              # f += M2 df2
         else:
             raise NotImplementedError("This time integrator is not implemented.")
+
+    def _inner_block(self, fs, k, h):
+        """k Euler steps of h; returns the last increments."""
+        d0 = None
+        for _ in range(k):
+            d0 = self.df(h)
+            for f, d in zip(fs, d0):
+                _axpy(f, d, 1.0)
+        return d0
 
     def step(self):
         t = dev.torch()
@@ -301,7 +329,17 @@ class Server:
         """Queue a request.  Requests that no state of the server could admit raise
         here instead of waiting forever: a memory need above the whole budget, more
         cells than a class may hold, or a step that does not divide t_final (the
-        reference's evolve_to_time check, src/solver.py:122-128)."""
+        reference's evolve_to_time check, src/solver.py:122-128), an integrator
+        this server does not run, or a schedule with too few levels for it."""
+        if req.integrator not in INTEGRATORS:
+            raise ValueError("request %r: unknown time integrator %r (known: %s)"
+                             % (req.rid, req.integrator, ", ".join(INTEGRATORS)))
+        levels = {"Euler": 0, "PFE": 1, "TPI2": 2, "TPI3": 3}[req.integrator]
+        if len(req.inner_steps) < levels or len(req.inner_dt) < levels or \
+                any(int(k) < 1 for k in req.inner_steps[:levels]):
+            raise ValueError("request %r: %s needs %d inner levels with at least one step each, "
+                             "got inner_steps=%r inner_dt=%r" % (req.rid, req.integrator, levels,
+                                                                 req.inner_steps, req.inner_dt))
         if req.nbytes() > self.budget:
             raise ValueError("request %r needs %d bytes, above the HBM budget of %d"
                              % (req.rid, req.nbytes(), self.budget))

@@ -601,3 +601,50 @@ def test_full_size_conservation_and_cooling(pkg, need_gpu, prec):
     assert np.all(np.abs(mom[:, 0]) <= 1e-12 * scale), mom[:, 0] / scale
     assert np.all(np.abs(mom[:, 1:4]) <= 1e-12 * scale[:, None] * sm.L), mom[:, 1:4]
     assert np.all(mom[:, 4] < 0), mom[:, 4]
+
+
+def test_grouped_launch_rejects_a_repeated_plan(pkg, need_gpu):
+    """The same operator twice in one grouped launch is an argument error, in the Python
+    wrapper and at the C-ABI, and leaves the plan usable with per-cell e."""
+    import ctypes
+    import torch
+    from proj_int_inelastic_amd import _lib
+    from proj_int_inelastic_amd import device as dev
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    from proj_int_inelastic_amd.collision.inelastic_collisions import collide_group
+    cfg = _cfg(2, 16, 8, 8, 2.5, 1.0, 1.0, 1.0)
+    sm = SpectralMesh(cfg)
+    c = FSInelasticVHSCollision(cfg, sm)
+    f = torch.from_numpy(_smooth(2, 16, sm.L, 3, seed=4)).cuda()
+    with pytest.raises(ValueError, match="twice"):
+        collide_group([c, c], [f, f[:2]], with_loss=True, es=[np.full(3, 0.4), np.full(2, 0.9)])
+    lib = _lib.load()
+    q = torch.empty_like(f)
+    P = ctypes.c_void_p
+    plans = (P * 2)(c._plan, c._plan)
+    fp = (P * 2)(P(f.data_ptr()), P(f.data_ptr()))
+    qp = (P * 2)(P(q.data_ptr()), P(q.data_ptr()))
+    nb = (ctypes.c_longlong * 2)(3, 3)
+    rc = lib.ib_collide_group(2, plans, fp, qp, nb, None, 1, 0.0, dev.stream_handle())
+    assert rc != 0 and "twice" in _lib.last_error()
+    e = np.array([0.3, 0.6, 0.95])
+    got = c.collide(f, with_loss=True, e=e).cpu().numpy()
+    for k in range(3):
+        ref = _oracle(2, 16, 8, 8, 2.5, e[k], e[k], 1.0).collide(f[k].cpu().numpy(), with_loss=True)
+        assert _err(got[k], ref) <= TOL64
+
+
+def test_axpy_chain_rounds_as_the_products(pkg, need_gpu):
+    """ib_axpy_chain: y + a (b (c x)) equals numpy's y + a * (b * (c * x)) bit for bit."""
+    import torch
+    from proj_int_inelastic_amd import _lib
+    from proj_int_inelastic_amd import device as dev
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=100003)
+    y = rng.normal(size=100003)
+    lib = _lib.load()
+    for a, b, c in [(1.7, 1.0, 1.0), (-1.0, 2.0, 1.0), (0.3, -1.0, 2.0), (1.0, 1 / 3, 7.1)]:
+        yd, xd = torch.from_numpy(y.copy()).cuda(), torch.from_numpy(x).cuda()
+        _lib.check(lib.ib_axpy_chain(yd.data_ptr(), xd.data_ptr(), a, b, c, yd.numel(),
+                                     dev.stream_handle()), "ib_axpy_chain")
+        assert np.array_equal(yd.cpu().numpy(), y + a * (b * (c * x))), (a, b, c)

@@ -116,3 +116,117 @@ def test_realtime_arrivals_match_offline(pkg, need_gpu):
         assert a.finished >= a.admitted >= a.arrival >= 0.0
     st = serving.latency_stats(live)
     assert st["n"] == len(reqs) and 0 < st["p50_s"] <= st["p99_s"]
+
+
+def _gauss_nd(d, n, L, seed):
+    rng = np.random.default_rng(seed)
+    dv = 2 * L / n
+    v = -L + (np.arange(n) + 0.5) * dv
+    g = np.meshgrid(*([v] * d), indexing="ij")
+    c = rng.normal(0, 0.1, (2, d))
+    f = sum(np.exp(-sum((g[a] - sg * (1.0 if a == 0 else 0.0) - c[k, a]) ** 2 for a in range(d)) / (2 * 0.5))
+            for k, sg in enumerate((1, -1)))
+    return f / (f.sum() * dv ** d)
+
+
+S3 = 3.624654714576
+# the uneven batch's shapes: (dim, N, M, s); Nr = N
+UNEVEN = [(2, 16, 8, 3.0), (2, 32, 8, S3), (2, 64, 8, 3.0), (3, 16, 38, 3.0), (3, 32, 38, S3)]
+
+
+def _uneven_reqs(Request, precision, t_final=0.25):
+    reqs = []
+    rng = np.random.default_rng(21)
+    for i, (d, n, m, s) in enumerate(UNEVEN):
+        L = 0.5 * (3 + math.sqrt(2)) * s
+        reqs.append(Request(rid=i, f0=_gauss_nd(d, n, L, 40 + i), t_final=t_final,
+                            e=float(rng.uniform(0.2, 1.0)), dim=d, n=n, nr=n, m=m, s=s,
+                            gamma=float(i % 2), precision=precision, integrator="TPI2",
+                            inner_steps=(2, 2), inner_dt=(0.01, 0.05), dt=0.25))
+    return reqs
+
+
+def test_uneven_tpi2_mix_against_oracle(pkg, need_gpu):
+    """The uneven batch's five shapes (2D 16^2 / 32^2 / 64^2, 3D 16^3 / 32^3, Nr = N, per-problem
+    e and gamma) advanced one TPI2 outer step together through the server: FP64 against the
+    oracle's tel_proj_step2 (1e-12), FP32 within 2e-5 of FP64 (relative L-inf of f)."""
+    from oracle import collision as OC
+    from oracle import integrators as OI
+    from oracle import quadrature as OQ
+    from proj_int_inelastic_amd import serving
+    r64 = serving.Server().run(_uneven_reqs(serving.Request, "fp64"), realtime=False)
+    r32 = serving.Server().run(_uneven_reqs(serving.Request, "fp32"), realtime=False)
+    for req in _uneven_reqs(serving.Request, "fp64"):
+        d, n, m, s = req.dim, req.n, req.m, req.s
+        rule = OQ.circle_rule(m) if d == 2 else OQ.lebedev_rule(m)
+        op = OC.OracleOperator(d, n, s, n, *rule, e=req.e, e0=req.e, gamma=req.gamma)
+        df = lambda f, h: h * op.collide(f, with_loss=True, workers=8)
+        ref = OI.tel_proj_step2(req.f0.copy(), df, 0.25, [2, 2], [0.01, 0.05])
+        scale = np.abs(ref).max()
+        e64 = np.abs(r64[req.rid].f - ref).max() / scale
+        e32 = np.abs(r32[req.rid].f - r64[req.rid].f).max() / scale
+        print("uneven", (d, n), "fp64 vs oracle", e64, "fp32 vs fp64", e32, flush=True)
+        assert e64 < 1e-12, ((d, n), e64)
+        assert e32 < 2e-5, ((d, n), e32)
+
+
+def _serving_mix(Request, precision):
+    """Serving-workload shapes: 1D x 2D 32^2 jobs with Nx in {64, 128, 256}, eps in {1e-2, 1e-3},
+    hard spheres, Sod-type data, PFE; homogeneous 2D 64^2 and 3D 16^3 / 32^3 TPI2 members."""
+    reqs = []
+    Ls = 0.5 * (3 + math.sqrt(2)) * S3
+    for i, (nx, kn) in enumerate([(64, 1e-2), (128, 1e-3), (256, 1e-2), (256, 1e-3)]):
+        reqs.append(Request(rid=i, f0=_riemann(nx, 32, Ls), t_final=1e-3, e=0.6 + 0.1 * i, dim=2,
+                            n=32, nr=32, m=8, s=S3, gamma=1.0, kn=kn, precision=precision,
+                            integrator="PFE", inner_steps=(2,), inner_dt=(6e-5,), dt=2.5e-4, xlen=1.0))
+    for j, (d, n, m, s, tf) in enumerate([(2, 64, 8, 3.0, 0.5), (3, 16, 38, 3.0, 0.75), (3, 32, 38, S3, 0.25),
+                                          (3, 16, 38, 3.0, 0.25)]):
+        L = 0.5 * (3 + math.sqrt(2)) * s
+        reqs.append(Request(rid=10 + j, f0=_gauss_nd(d, n, L, 70 + j), t_final=tf, e=0.35 + 0.15 * j,
+                            dim=d, n=n, nr=n, m=m, s=s, gamma=float(j % 2), precision=precision,
+                            integrator="TPI2", inner_steps=(2, 2), inner_dt=(0.01, 0.05), dt=0.25))
+    return reqs
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_serving_workload_shapes_alone_equals_mix(pkg, need_gpu, precision):
+    from proj_int_inelastic_amd import serving
+    mix = serving.Server().run(_serving_mix(serving.Request, precision), realtime=False)
+    for r in _serving_mix(serving.Request, precision):
+        alone = serving.Server().run([r], realtime=False)[r.rid]
+        a, b = alone.f, mix[r.rid].f
+        err = np.abs(a - b).max() / np.abs(a).max()
+        assert err < 1e-12, (precision, r.rid, err)
+        assert abs(alone.t - r.t_final) < 1e-12
+
+
+def test_server_tpi3_matches_solver(pkg, need_gpu):
+    """A TPI3 class of the server equals the solver's tel_proj_step3 on the same problems."""
+    import types
+    from proj_int_inelastic_amd import serving
+    from proj_int_inelastic_amd.boltzmann import BoltzmannSolver0D
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    from proj_int_inelastic_amd.solution import Solution
+    L = 0.5 * (3 + math.sqrt(2)) * 3.0
+    es = [0.4, 0.85]
+    reqs = [serving.Request(rid=i, f0=_gauss2(16, L, 50 + i), t_final=1.0, e=e, n=16, nr=16, m=8, s=3.0,
+                            gamma=1.0, integrator="TPI3", inner_steps=(3, 3, 3), inner_dt=(0.01, 0.05, 0.1),
+                            dt=0.5) for i, e in enumerate(es)]
+    res = serving.Server().run(list(reqs), realtime=False)
+    cfg = {"collision-model": {"dim": 2, "e": 1.0, "e0": 1.0, "gamma": 1.0},
+           "velocity-mesh": {"nv": 16, "s": 3.0, "nr": 16}, "circle-quad-rule": {"nphi": 8}}
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm)
+    sol = Solution(types.SimpleNamespace(dim_x=0, dim_v=2, xmin=0.0, xmax=1.0, l=sm.L, nx=1, nv=16))
+    sol.state.f = np.stack([r.f0 for r in reqs])
+    solver = BoltzmannSolver0D(kn=1.0, collision_operator=lambda f: coll.collide(f, with_loss=True,
+                                                                                 e=np.array(es)))
+    solver.time_integrator = "TPI3"
+    solver.inner_steps = [3, 3, 3]
+    solver.inner_dt = [0.01, 0.05, 0.1]
+    solver.dt = 0.5
+    solver.evolve_to_time(sol, 1.0)
+    fg = sol.state.get_f_global()
+    for i in range(2):
+        err = np.abs(res[i].f - fg[i]).max() / np.abs(fg[i]).max()
+        assert err < 1e-12, (i, err)

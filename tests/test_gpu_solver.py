@@ -187,6 +187,41 @@ def test_tpi2_homogeneous_batch_per_problem_e(pkg, need_gpu):
     assert err < 1e-12, err
 
 
+def test_tpi3_homogeneous_batch_per_problem_e(pkg, need_gpu):
+    """TPI3 (boltzmann.py:142-158) with the reference's default schedule shape: K = (3, 3, 3),
+    inner dt (0.01, 0.05, 0.1), so M0 = 2, M1 = -1 (df2 = M1 df1 as written) and M2 = 2 at
+    outer dt 0.5; three problems with their own e, against the oracle's tel_proj_step3."""
+    from proj_int_inelastic_amd.boltzmann import BoltzmannSolver0D
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    from proj_int_inelastic_amd.solution import Solution
+    N, es = 16, np.array([0.25, 0.7, 0.95])
+    cfg = _coll_cfg(N, 16, 8, 2.5, 1.0, 1.0)
+    sm = SpectralMesh(cfg)
+    coll = FSInelasticVHSCollision(cfg, sm)
+    op = [OC.OracleOperator(2, N, 2.5, 16, *OQ.circle_rule(8), e=x, e0=x, gamma=1.0) for x in es]
+    v, dv = op[0].v_grids(), op[0].delta
+    f0 = np.stack([_two_gauss(v, dv, seed=10 + s) for s in range(3)])
+    sol = Solution(types.SimpleNamespace(dim_x=0, dim_v=2, xmin=0.0, xmax=1.0, l=sm.L, nx=1, nv=N))
+    sol.state.f = f0.copy()
+    solver = BoltzmannSolver0D(kn=1.0, collision_operator=lambda f: coll.collide(f, with_loss=True, e=es))
+    solver.time_integrator = "TPI3"
+    solver.inner_steps = [3, 3, 3]
+    solver.inner_dt = [0.01, 0.05, 0.1]
+    solver.dt = 0.5
+    solver.evolve_to_time(sol, 1.0)
+    fg = sol.state.get_f_global()
+
+    def df(f, h):
+        return np.stack([h * op[c].collide(f[c], with_loss=True) for c in range(3)])
+
+    f = f0.copy()
+    for _ in range(2):
+        f = OI.tel_proj_step3(f, df, 0.5, [3, 3, 3], [0.01, 0.05, 0.1])
+    err = np.abs(fg - f).max() / np.abs(f).max()
+    print("TPI3 rel Linf", err)
+    assert err < 1e-12, err
+
+
 def test_graph_replay_matches_eager(pkg, need_gpu):
     import torch
     from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh

@@ -234,3 +234,36 @@ def test_nyquist_term_completes_symmetrized(N, nr, M, seed):
     qfix = qs + (np.fft.ifftn(term) / op.sphr_fac).real
     assert np.abs(qs - qc).max() > 1e-4 * np.abs(qc).max()  # the term matters here
     assert np.abs(qfix - qc).max() <= 1e-14 * np.abs(qc).max()
+
+
+def test_restitution_profile_values():
+    """e(r) = (e0 - e)/2 tanh(r - R/2) + (e0 + e)/2 (inelastic_collisions.py:77-79), pinned by its
+    values: the midpoint of the blend at r = R/2, the odd symmetry of tanh about it, and the limits
+    e (r << R/2) and e0 (r >> R/2)."""
+    from oracle.collision import e_profile
+    R, e, e0 = 6.0, 0.35, 0.95
+    assert e_profile(np.array([R / 2]), R, e, e0)[0] == 0.5 * (e + e0)
+    t = np.array([0.1, 0.7, 2.5])
+    np.testing.assert_allclose(e_profile(R / 2 + t, R, e, e0) + e_profile(R / 2 - t, R, e, e0), e + e0,
+                               rtol=0, atol=1e-15)
+    assert abs(e_profile(np.array([R / 2 + 30.0]), R, e, e0)[0] - e0) < 1e-15
+    assert abs(e_profile(np.array([R / 2 - 30.0]), R, e, e0)[0] - e) < 1e-15
+    # constant e when e0 = e (the 3D configuration sets both)
+    assert np.all(e_profile(np.linspace(0, R, 7), R, 0.6, 0.6) == 0.6)
+    # slope at the midpoint: d/dr = (e0 - e)/2
+    hstep = 1e-6
+    slope = (e_profile(np.array([R / 2 + hstep]), R, e, e0)[0]
+             - e_profile(np.array([R / 2 - hstep]), R, e, e0)[0]) / (2 * hstep)
+    assert abs(slope - 0.5 * (e0 - e)) < 1e-8
+
+
+def test_spherical_mean_kernel_values():
+    """sp = J0 (2D) and sin x / x (3D) (inelastic_collisions.py:42-47), pinned by known values."""
+    from oracle.collision import sp_func
+    j0_zero = 2.404825557695773
+    assert abs(sp_func(2, np.array([j0_zero]))[0]) < 1e-15
+    assert sp_func(2, np.array([0.0]))[0] == 1.0
+    assert abs(sp_func(2, np.array([1.0]))[0] - 0.7651976865579666) < 1e-15
+    assert sp_func(3, np.array([0.0]))[0] == 1.0
+    assert abs(sp_func(3, np.array([np.pi]))[0]) < 1e-16
+    assert abs(sp_func(3, np.array([1.0]))[0] - np.sin(1.0)) < 1e-16
