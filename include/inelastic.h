@@ -183,6 +183,30 @@ int ib_tvd_axis(const double* f, double* df, long long outer, int n, long long i
                 const double* v, long long vmod, double dtdx, int order, int limiter,
                 int bc_lower, int bc_upper, int accumulate, void* stream);
 
+/* Wave limiting alone: out[i, j] = the limited wave of interface i, column j, of the (n, inner)
+ * float64 waves, for rows 1 .. n - 2 (rows 0 and n - 1 copied): theta = (s > 0 ? W[i-1] W[i] :
+ * W[i+1] W[i]) / W[i]^2 with the masked division (0 where numpy's masked true_divide masks),
+ * W~ = phi(theta) W where W != 0, limiter 0 = minmod, 1 = MC.  The speed of (i, j) is
+ * s[((i - 1) inner + j) % smod] with smod 1, inner or (n - 2) inner; out must not alias wave.
+ * Replaces src/limiters/tvd.py:6-27 (limit; its dtdx feeds only the cfl argument, which
+ * neither limiter reads). */
+int ib_tvd_limit(const double* wave, double* out, long long n, long long inner, const double* s,
+                 long long smod, int limiter, void* stream);
+
+/* Ghost arrays.  ib_ghost_embed copies f (n0, inner) [num_dim_x = 1; n1 = 1] or (n0, n1, inner)
+ * [num_dim_x = 2] into the interior of fbc (n0 + 2g, inner) or (n0 + 2g, n1 + 2g, inner), g =
+ * num_ghost; extract != 0 copies the interior of fbc back into f.  Replaces src/state.py:109-122
+ * (get_fbc_from_f) and :95-107 (set_f_from_fbc). */
+int ib_ghost_embed(const double* f, double* fbc, int num_dim_x, long long n0, long long n1,
+                   long long inner, int num_ghost, int extract, void* stream);
+
+/* Fill the num_ghost ghost rows on each side of the middle axis of fbc (outer, n + 2 g, inner) in
+ * place from that axis's interior rows: bc 1 (BC.extrap) repeats the first / last interior row,
+ * 2 (BC.periodic) wraps, 0 leaves that side unchanged.  Replaces src/solver.py:67-105 (_bc_lower,
+ * _bc_upper); one call per x dimension gives _apply_bcs (:57-65). */
+int ib_ghost_fill(double* fbc, long long outer, long long n, long long inner, int num_ghost, int bc_lower,
+                  int bc_upper, void* stream);
+
 /* y <- y + a x over n float64 values.  Replaces the `state.f += M * df` updates of
  * src/boltzmann.py:84,126-128,137-140,153-158. */
 int ib_axpy(double* y, const double* x, double a, long long n, void* stream);

@@ -97,6 +97,12 @@ int launch_tvd1d(const double* f, double* df, long long outer, int nx, long long
                  const double* v1, long long vmod, double dtdx, int order, int limiter, int bc_lo,
                  int bc_hi, int accumulate, hipStream_t st);
 int launch_axpy(double* y, const double* x, double a, long long n, hipStream_t st);
+int launch_tvd_limit(const double* w, double* out, long long n, long long inner, const double* s,
+                     long long smod, int limiter, hipStream_t st);
+int launch_embed(const double* a, double* b, long long n0, long long n1, long long inner, int g,
+                 int pad1, int extract, hipStream_t st);
+int launch_ghost_axis(double* a, long long outer, long long n, long long inner, int g, int bc_lo,
+                      int bc_hi, hipStream_t st);
 int launch_axpy_chain(double* y, const double* x, double a, double b, double c, long long n,
                       hipStream_t st);
 int launch_df_combine(const double* adv, const double* q, double* out, double dt, double kn,
@@ -570,6 +576,54 @@ int ib_fast_sweep(double* f, const double* gain, const double* rho, const double
     return IB_ERR_ARG;
   }
   return launch_fast_sweep(f, gain, rho, vx, nx, nvr, nvc, dx, nu, err, (hipStream_t)stream);
+}
+
+int ib_tvd_limit(const double* wave, double* out, long long n, long long inner, const double* s,
+                 long long smod, int limiter, void* stream) {
+  if (n < 0 || inner < 0 || smod < 1 || (limiter != 0 && limiter != 1) ||
+      (n * inner > 0 && (!wave || !out || !s))) {
+    set_error("invalid tvd_limit arguments");
+    return IB_ERR_ARG;
+  }
+  if (n > 2 && smod > 1 && smod != inner && smod != (n - 2) * inner) {
+    set_error("tvd_limit: speeds must be one value, one per column or one per limited wave");
+    return IB_ERR_ARG;
+  }
+  if (n * inner > 0 && out == wave) {
+    set_error("tvd_limit: out must not alias wave");
+    return IB_ERR_ARG;
+  }
+  if (n * inner == 0) return IB_OK;
+  return launch_tvd_limit(wave, out, n, inner, s, smod, limiter, (hipStream_t)stream);
+}
+
+int ib_ghost_embed(const double* f, double* fbc, int num_dim_x, long long n0, long long n1,
+                   long long inner, int num_ghost, int extract, void* stream) {
+  if ((num_dim_x != 1 && num_dim_x != 2) || n0 < 0 || n1 < 0 || inner < 0 || num_ghost < 0 ||
+      (num_dim_x == 1 && n1 != 1) || (n0 * n1 * inner > 0 && (!f || !fbc))) {
+    set_error(num_dim_x == 1 || num_dim_x == 2 ? "invalid ghost_embed arguments"
+                                               : "Assumption (1 <= num_dim <= 2) violated.");
+    return IB_ERR_ARG;
+  }
+  if (n0 * n1 * inner == 0) return IB_OK;
+  return extract ? launch_embed(fbc, (double*)f, n0, n1, inner, num_ghost, num_dim_x == 2, 1,
+                                (hipStream_t)stream)
+                 : launch_embed(f, fbc, n0, n1, inner, num_ghost, num_dim_x == 2, 0, (hipStream_t)stream);
+}
+
+int ib_ghost_fill(double* fbc, long long outer, long long n, long long inner, int num_ghost, int bc_lower,
+                  int bc_upper, void* stream) {
+  if (outer < 0 || n < 0 || inner < 0 || num_ghost < 0 || (outer * n * inner > 0 && !fbc) ||
+      bc_lower < 0 || bc_lower > 2 || bc_upper < 0 || bc_upper > 2) {
+    set_error("invalid ghost_fill arguments");
+    return IB_ERR_ARG;
+  }
+  if ((bc_lower == 2 || bc_upper == 2) && n < num_ghost) {
+    set_error("periodic ghost cells need at least num_ghost interior cells");
+    return IB_ERR_ARG;
+  }
+  if (outer * inner * num_ghost == 0 || n == 0) return IB_OK;
+  return launch_ghost_axis(fbc, outer, n, inner, num_ghost, bc_lower, bc_upper, (hipStream_t)stream);
 }
 
 int ib_axpy(double* y, const double* x, double a, long long n, void* stream) {

@@ -114,6 +114,65 @@ __global__ __launch_bounds__(TVD_THREADS) void k_tvd1d(
   }
 }
 
+// Wave limiting alone (src/limiters/tvd.py:6-27) on (n, inner) waves: rows 1 .. n - 2 are
+// limited with the speed s[((row - 1) inner + column) % smod], rows 0 and n - 1 copied.
+// Out of place: every output row reads the three input rows around it.
+__global__ void k_tvd_limit(const double* __restrict__ w, double* __restrict__ out, long long n,
+                            long long inner, const double* __restrict__ s, long long smod,
+                            int limiter) {
+  const long long tot = n * inner;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < tot;
+       g += (long long)gridDim.x * blockDim.x) {
+    const long long i = g / inner, j = g - i * inner;
+    if (i == 0 || i == n - 1) {
+      out[g] = w[g];
+      continue;
+    }
+    const double spos = s[((i - 1) * inner + j) % smod] > 0.0 ? 1.0 : 0.0;
+    out[g] = limited(w[g - inner], w[g], w[g + inner], spos, limiter);
+  }
+}
+
+// Ghost arrays of src/solver.py:51-105 and src/state.py:95-122.  k_embed copies f (n0[, n1],
+// inner) into the interior of fbc (n0 + 2g[, n1 + 2g], inner), or back (extract); k_ghost_axis
+// fills the g ghost rows of each side of the middle axis of (outer, n + 2g, inner) in place from
+// that axis's interior rows: extrapolation copies the first / last interior row, periodic wraps
+// (lower ghosts = the last g interior rows, upper = the first g); bc 0 leaves a side as it is.
+__global__ void k_embed(const double* __restrict__ a, double* __restrict__ b, long long n0, long long n1,
+                        long long inner, int g, int pad1, int extract) {
+  const long long tot = n0 * n1 * inner;
+  const long long m1 = pad1 ? n1 + 2LL * g : n1;  // pad1: axis 1 is padded too (2D space)
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long k = t % inner, ij = t / inner, j = ij % n1, i = ij / n1;
+    const long long u = ((i + g) * m1 + (pad1 ? j + g : j)) * inner + k;
+    if (extract) b[t] = a[u];
+    else b[u] = a[t];
+  }
+}
+
+__global__ void k_ghost_axis(double* __restrict__ a, long long outer, long long n, long long inner, int g,
+                             int bc_lo, int bc_hi) {
+  const long long tot = outer * 2LL * g * inner;
+  const long long rows = n + 2LL * g;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long k = t % inner, r = (t / inner) % (2LL * g), o = t / (inner * 2LL * g);
+    const bool lower = r < g;
+    const int bc = lower ? bc_lo : bc_hi;
+    if (bc == 0) continue;
+    long long dst, src;
+    if (lower) {
+      dst = r;
+      src = bc == 1 ? g : n + r;            // periodic: interior row n - g + r
+    } else {
+      dst = n + g + (r - g);
+      src = bc == 1 ? n + g - 1 : g + (r - g);
+    }
+    a[(o * rows + dst) * inner + k] = a[(o * rows + src) * inner + k];
+  }
+}
+
 __global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, double a, long long n) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
@@ -341,6 +400,30 @@ static unsigned grid_for(long long n) {
   if (b > 65536) b = 65536;
   if (b < 1) b = 1;
   return (unsigned)b;
+}
+
+int launch_tvd_limit(const double* w, double* out, long long n, long long inner, const double* s,
+                     long long smod, int limiter, hipStream_t st) {
+  hipLaunchKernelGGL(k_tvd_limit, dim3(grid_for(n * inner)), dim3(256), 0, st, w, out, n, inner, s,
+                     smod, limiter);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+int launch_embed(const double* a, double* b, long long n0, long long n1, long long inner, int g,
+                 int pad1, int extract, hipStream_t st) {
+  hipLaunchKernelGGL(k_embed, dim3(grid_for(n0 * n1 * inner)), dim3(256), 0, st, a, b, n0, n1, inner, g,
+                     pad1, extract);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+int launch_ghost_axis(double* a, long long outer, long long n, long long inner, int g, int bc_lo,
+                      int bc_hi, hipStream_t st) {
+  hipLaunchKernelGGL(k_ghost_axis, dim3(grid_for(outer * 2LL * g * inner)), dim3(256), 0, st, a, outer,
+                     n, inner, g, bc_lo, bc_hi);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
 }
 
 int launch_axpy(double* y, const double* x, double a, long long n, hipStream_t st) {

@@ -1,6 +1,13 @@
 """Solver base (reference ``src/solver.py``): BC enum, fixed-dt
-``evolve_to_time`` (:107-143), and the ghost-cell conventions (:51-105), which
-the device stencil kernel applies on the fly (no ghost array is copied)."""
+``evolve_to_time`` (:107-143), and the ghost-cell conventions (:51-105).  The
+transport step's stencil kernel applies the ghost cells on the fly and never
+builds a ghost array; ``_allocate_bc_arrays`` / ``_apply_bcs`` / ``_bc_lower`` /
+``_bc_upper`` build ``fbc`` on the device for callers that use it (HIP kernels
+ib_ghost_embed and ib_ghost_fill)."""
+import numpy as np
+
+from . import _lib
+from . import device as dev
 
 
 class BC:
@@ -37,6 +44,47 @@ class Solver(object):
 
     def setup(self, solution):
         self._is_set_up = True
+
+    # -- solver.py:51-56
+    def _allocate_bc_arrays(self, state):
+        t = dev.require_gpu()
+        nd = self.num_dim_x
+        shape = [n + 2 * self.num_ghost for n in state.f.shape[:nd]] + list(state.f.shape[nd:])
+        self.fbc = t.zeros(shape, dtype=t.float64, device="cuda")
+        self._apply_bcs(state)
+
+    # -- solver.py:57-65
+    def _apply_bcs(self, state):
+        self.fbc = state.get_fbc_from_f(self.num_ghost, self.fbc)
+        for idim in range(self.num_dim_x):
+            self._bc_lower(self.bc_lower[idim], self.fbc, axis=idim)
+            self._bc_upper(self.bc_upper[idim], self.fbc, axis=idim)
+
+    def _ghost_call(self, array, axis, lo, hi):
+        g = self.num_ghost
+        shape = tuple(array.shape)
+        if not (dev.is_device_tensor(array) and array.is_contiguous()):
+            raise ValueError("the ghost-extended array must be a contiguous device tensor")
+        outer = int(np.prod(shape[:axis], dtype=np.int64))
+        inner = int(np.prod(shape[axis + 1:], dtype=np.int64))
+        _lib.check(_lib.load().ib_ghost_fill(array.data_ptr(), outer, shape[axis] - 2 * g, inner, g, lo, hi,
+                                             dev.stream_handle()), "ib_ghost_fill")
+
+    @staticmethod
+    def _bc_code(bc_type, side):
+        if bc_type in (BC.extrap, BC.periodic):
+            return bc_type
+        if bc_type is None:
+            raise Exception("%s boundary condition not specified." % side)
+        raise NotImplementedError("Boundary condition %s not implemented" % bc_type)
+
+    # -- solver.py:67-85 (the axis the reference selects with np.rollaxis is an argument here)
+    def _bc_lower(self, bc_type, array, axis=0):
+        self._ghost_call(array, axis, self._bc_code(bc_type, "Lower"), 0)
+
+    # -- solver.py:87-105
+    def _bc_upper(self, bc_type, array, axis=0):
+        self._ghost_call(array, axis, 0, self._bc_code(bc_type, "Upper"))
 
     def _bc_codes(self, idim=0):
         lo, hi = self.bc_lower[idim], self.bc_upper[idim]

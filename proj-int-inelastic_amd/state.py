@@ -47,6 +47,34 @@ class State(object):
     def get_f_global(self):
         return self.f.detach().cpu().numpy()
 
+    def _ghost_dims(self, num_ghost, fbc):
+        nd = self.grid.num_dim_x or 0
+        if nd not in (1, 2):
+            raise Exception("Assumption (1 <= num_dim <= 2) violated.")
+        f = self.f
+        n0 = int(f.shape[0])
+        n1 = int(f.shape[1]) if nd == 2 else 1
+        inner = int(np.prod(f.shape[nd:], dtype=np.int64))
+        want = (n0 + 2 * num_ghost,) + ((n1 + 2 * num_ghost,) if nd == 2 else ()) + tuple(f.shape[nd:])
+        if tuple(fbc.shape) != want or not dev.is_device_tensor(fbc) or not fbc.is_contiguous():
+            raise ValueError("fbc must be a contiguous float64 device array of shape %r" % (want,))
+        return nd, n0, n1, inner
+
+    # -- state.py:109-122
+    def get_fbc_from_f(self, num_ghost, fbc):
+        """Copy f into the interior of the ghost-extended array fbc (a device tensor) and return it."""
+        nd, n0, n1, inner = self._ghost_dims(num_ghost, fbc)
+        _lib.check(_lib.load().ib_ghost_embed(self.f.data_ptr(), fbc.data_ptr(), nd, n0, n1, inner,
+                                              int(num_ghost), 0, dev.stream_handle()), "ib_ghost_embed")
+        return fbc
+
+    # -- state.py:95-107
+    def set_f_from_fbc(self, num_ghost, fbc):
+        """Set f from the interior of the ghost-extended array fbc."""
+        nd, n0, n1, inner = self._ghost_dims(num_ghost, fbc)
+        _lib.check(_lib.load().ib_ghost_embed(self.f.data_ptr(), fbc.data_ptr(), nd, n0, n1, inner,
+                                              int(num_ghost), 1, dev.stream_handle()), "ib_ghost_embed")
+
     def _vel_moments(self):
         nx = self.grid.num_dim_x or 0
         dv_dims = self.grid.dimensions[nx:]

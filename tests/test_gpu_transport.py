@@ -224,3 +224,106 @@ def test_axpy_and_df_combine_bit_exact(pkg, need_gpu, n, off):
     c = (0.013 * q[off:]) / 0.0071
     assert np.array_equal(out.cpu().numpy()[off:], adv[off:] + c)
     assert np.array_equal(o2.cpu().numpy()[off:], c)
+
+
+def test_tvd_limit_golden_bit_exact(pkg, need_gpu):
+    """limiters.tvd.limit (tvd.py:6-27) on the golden waves written from the reference's own
+    module: bit-exact, in place on numpy input, and the same on a device tensor."""
+    import torch
+    from proj_int_inelastic_amd.limiters import tvd
+    w = G["limit_wave"].copy()
+    out = tvd.limit(w, G["limit_s"], tvd.minmod, 0.1)
+    assert out is w and np.array_equal(w, G["limit_out"])
+    wd = torch.from_numpy(G["limit_wave"].copy()).cuda()
+    od = tvd.limit(wd, G["limit_s"], tvd.minmod, 0.1)
+    assert od is wd and np.array_equal(wd.cpu().numpy(), G["limit_out"])
+
+
+@pytest.mark.parametrize("limiter", ["minmod", "mc"])
+@pytest.mark.parametrize("s_kind", ["scalar", "column", "full", "row1"])
+def test_tvd_limit_matches_oracle(pkg, need_gpu, limiter, s_kind):
+    """Random waves with exact zeros, tiny values next to large ones (the masked division's
+    |num| tiny >= |den| case) and sign changes; speeds of every broadcast form."""
+    from proj_int_inelastic_amd.limiters import tvd
+    rng = np.random.default_rng(5)
+    n, a, b = 37, 4, 5
+    w = rng.normal(size=(n, a, b))
+    w[rng.random(w.shape) < 0.15] = 0.0
+    w[5, 1, 2], w[6, 1, 2] = 1e-300, 1e300
+    w[9, 0, 0], w[10, 0, 0] = 5e-320, -3.0
+    s = {"scalar": np.array(-0.5), "column": rng.normal(size=(a, b)), "full": rng.normal(size=(n - 2, a, b)),
+         "row1": rng.normal(size=(1, a, b))}[s_kind]
+    ref = T.limit(w, s, getattr(T, limiter), 0.3)
+    got = tvd.limit(w.copy(), s, getattr(tvd, limiter), 0.3)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(got[0], w[0]) and np.array_equal(got[-1], w[-1])
+
+
+def test_tvd_limit_small_and_errors(pkg, need_gpu):
+    import torch
+    from proj_int_inelastic_amd import _lib
+    from proj_int_inelastic_amd import device as dev
+    from proj_int_inelastic_amd.limiters import tvd
+    w = np.array([[1.0], [2.0]])
+    assert np.array_equal(tvd.limit(w.copy(), 1.0, tvd.minmod, 0.1), w)   # no interior wave
+    with pytest.raises(NotImplementedError):
+        tvd.limit(w.copy(), 1.0, lambda r, c: r, 0.1)
+    x = torch.zeros(12, dtype=torch.float64, device="cuda")
+    sp = torch.ones(3, dtype=torch.float64, device="cuda")
+    lib = _lib.load()
+    assert lib.ib_tvd_limit(x.data_ptr(), x.data_ptr(), 4, 3, sp.data_ptr(), 3, 0, dev.stream_handle()) != 0
+    assert "alias" in _lib.last_error()
+    y = torch.empty_like(x)
+    assert lib.ib_tvd_limit(x.data_ptr(), y.data_ptr(), 4, 3, sp.data_ptr(), 2, 0, dev.stream_handle()) != 0
+    assert lib.ib_tvd_limit(x.data_ptr(), y.data_ptr(), 4, 3, sp.data_ptr(), 3, 7, dev.stream_handle()) != 0
+
+
+@pytest.mark.parametrize("g", [1, 2, 3])
+@pytest.mark.parametrize("bcs", [(1, 1), (2, 2), (1, 2), (2, 1)])
+def test_ghost_arrays_1d_match_oracle(pkg, need_gpu, g, bcs):
+    """Solver._allocate_bc_arrays / _apply_bcs and State.get_fbc_from_f / set_f_from_fbc
+    (solver.py:51-105, state.py:95-122) on a 1D x 2D state against the oracle's ghost fill."""
+    import types
+    import torch
+    from proj_int_inelastic_amd.boltzmann import BoltzmannSolver1D
+    from proj_int_inelastic_amd.solution import Solution
+    nx, N = 9, 4
+    sol = Solution(types.SimpleNamespace(dim_x=1, dim_v=2, xmin=0.0, xmax=1.0, l=4.0, nx=nx, nv=N))
+    f = np.random.default_rng(g).normal(size=(nx, N, N))
+    sol.state.f = f
+    s = BoltzmannSolver1D(kn=1.0)
+    s.num_ghost = g
+    s.bc_lower[0], s.bc_upper[0] = bcs
+    s._allocate_bc_arrays(sol.state)
+    ref = T.fill_ghosts(f, bcs[0], bcs[1], g)
+    assert np.array_equal(s.fbc.cpu().numpy(), ref)
+    # set_f_from_fbc takes the interior back
+    s.fbc.mul_(2.0)
+    sol.state.set_f_from_fbc(g, s.fbc)
+    assert np.array_equal(sol.state.get_f_global(), 2.0 * f)
+
+
+def test_ghost_arrays_2d_corners(pkg, need_gpu):
+    """2D space: x ghosts first, then y ghosts over every x row (corners = y-fill of x-ghost rows),
+    as solver.py:57-65 applies them."""
+    import types
+    from proj_int_inelastic_amd.boltzmann import BoltzmannSolver2D
+    from proj_int_inelastic_amd.solution import Solution
+    nx = ny = 6
+    N = 3
+    sol = Solution(types.SimpleNamespace(dim_x=2, dim_v=2, xmin=0.0, xmax=1.0, l=4.0, nx=nx, nv=N))
+    f = np.random.default_rng(3).normal(size=(nx, ny, N, N))
+    sol.state.f = f
+    s = BoltzmannSolver2D(kn=1.0)
+    s.bc_lower = [1, 2]
+    s.bc_upper = [2, 1]
+    s._allocate_bc_arrays(sol.state)
+    g = s.num_ghost
+    ref = np.zeros((nx + 2 * g, ny + 2 * g, N, N))
+    ref[g:-g, g:-g] = f
+    ref[:, g:-g] = T.fill_ghosts(f, 1, 2, g)
+    ref = np.moveaxis(T.fill_ghosts(np.moveaxis(ref[:, g:-g], 1, 0), 2, 1, g), 0, 1)
+    assert np.array_equal(s.fbc.cpu().numpy(), ref)
+    s.bc_upper[1] = None
+    with pytest.raises(Exception, match="Upper boundary condition not specified"):
+        s._apply_bcs(sol.state)
