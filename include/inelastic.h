@@ -76,11 +76,21 @@ int ib_plan_set_params(ib_plan* plan, int n_param, const double* e_r, const doub
 int ib_plan_destroy(ib_plan* plan);
 
 /* Tuning knobs of a plan (0 keeps the automatic choice): quadrature pairs per
- * staged chunk, cells per fused launch, scratch budget in bytes of the staged
- * engine.  New (the reference's FFTW plans have no such knobs); used by the
+ * staged chunk, cells per fused launch, scratch budget in bytes (the staged engine's
+ * sub-batches and chunks, and the cells per fused launch, are cut to fit it).  New (the reference's FFTW plans have no such knobs); used by the
  * autotuner. */
 int ib_plan_set_tuning(ib_plan* plan, long long chunk_pairs, long long fused_cells,
                        long long scratch_budget);
+
+/* Device scratch of a plan: *held = the bytes its buffers hold now; *need = the bytes they
+ * will hold after a call of `cells` cells (ib_collide for grouped = 0, a segment of
+ * ib_collide_group for grouped = 1) under the current tuning: exact for the fused engines,
+ * an upper bound for the staged engine (its pair-sum split is sized by the whole group).
+ * Buffers only grow, so need >= held.  Either pointer may be NULL.  Fused launches are cut so
+ * that a sub-batch's scratch stays within the plan's scratch budget (at least one cell).
+ * New: admission control of a server that shares one GPU among many plans (the reference
+ * runs one operator per process). */
+int ib_plan_scratch_need(ib_plan* plan, long long cells, int grouped, long long* need, long long* held);
 
 /* HIP-event timing of the dominant kernel of ib_collide: the fused FP32 kernel
  * (cells counted per launch) or, on the staged path, the slab pass (cell x

@@ -65,6 +65,7 @@ SIGNATURES = {
     "ib_axpy": (_I, [_P, _P, _D, _LL, _P]),
     "ib_axpy_chain": (_I, [_P, _P, _D, _D, _D, _LL, _P]),
     "ib_tvd_limit": (_I, [_P, _P, _LL, _LL, _P, _LL, _I, _P]),
+    "ib_plan_scratch_need": (_I, [_P, _LL, _I, _P, _P]),
     "ib_ghost_embed": (_I, [_P, _P, _I, _LL, _LL, _LL, _I, _I, _P]),
     "ib_ghost_fill": (_I, [_P, _LL, _LL, _LL, _I, _I, _I, _P]),
     "ib_df_combine": (_I, [_P, _P, _P, _D, _D, _LL, _P]),

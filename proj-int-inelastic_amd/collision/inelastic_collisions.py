@@ -118,9 +118,18 @@ class FSInelasticVHSCollision(BaseCollision):
 
     def set_tuning(self, chunk_pairs=0, fused_cells=0, scratch_budget=0):
         """Plan knobs (0 = automatic): pairs per staged chunk, cells per fused launch,
-        staged scratch budget in bytes."""
+        scratch budget in bytes (staged sub-batches and fused launches are cut to fit it)."""
         _lib.check(_lib.load().ib_plan_set_tuning(self._plan, int(chunk_pairs), int(fused_cells),
                                                   int(scratch_budget)), "ib_plan_set_tuning")
+
+    def scratch_need(self, cells, grouped=True):
+        """(need, held): device bytes the plan's scratch will hold after a call of `cells`
+        cells (a segment of collide_group when grouped) and the bytes it holds now."""
+        need, held = ctypes.c_longlong(), ctypes.c_longlong()
+        _lib.check(_lib.load().ib_plan_scratch_need(self._plan, int(cells), int(bool(grouped)),
+                                                    ctypes.byref(need), ctypes.byref(held)),
+                   "ib_plan_scratch_need")
+        return need.value, held.value
 
     def set_timing(self, enable=True):
         """Record HIP events around the dominant kernel of every collide call."""

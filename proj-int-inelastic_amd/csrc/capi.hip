@@ -23,6 +23,15 @@ size_t arena_left() {
   return at >= g_arena->size ? 0 : g_arena->size - at;
 }
 
+long long fused_sub(const Plan* P, long long batch, long long dflt, size_t per_cell) {
+  long long cap = P->fused_cells > 0 ? P->fused_cells : dflt;
+  if (!g_arena && per_cell > 0)
+    cap = std::min<long long>(cap, std::max<long long>(1, (long long)(P->scratch_budget / per_cell)));
+  if (batch <= cap) return std::max<long long>(batch, 1);
+  const long long launches = (batch + cap - 1) / cap;
+  return (batch + launches - 1) / launches;
+}
+
 int DevBuf::ensure(size_t want) {
   if (want <= bytes) return IB_OK;
   if (g_arena) {  // static-scratch call: carve, never allocate
@@ -292,6 +301,31 @@ int ib_plan_set_params(ib_plan* p, int n_param, const double* e_r, const double*
   int rc = set_params(p, n_param, e_r, gamma, st, p->d_wr);
   if (rc) return rc;
   IB_HIP(hipStreamSynchronize(st));
+  return IB_OK;
+}
+
+int ib_plan_scratch_need(ib_plan* p, long long cells, int grouped, long long* need, long long* held) {
+  if (!p || cells < 0 || (!need && !held)) {
+    set_error("invalid scratch_need arguments");
+    return IB_ERR_ARG;
+  }
+  const DevBuf* bufs[4] = {&p->fhat, &p->acc, &p->work, &p->aux};
+  if (held) {
+    long long h = 0;
+    for (const DevBuf* b : bufs) h += (long long)b->bytes;
+    *held = h;
+  }
+  if (need) {
+    ScratchSizes z;
+    if (cells > 0 && !fused_sizes(p, cells, &z)) {
+      const int rc = staged_sizes(p, cells, grouped, &z);
+      if (rc) return rc;
+    }
+    const size_t want[4] = {z.fhat, z.acc, z.work, z.aux};
+    long long n = 0;
+    for (int i = 0; i < 4; ++i) n += (long long)std::max(want[i], bufs[i]->bytes);
+    *need = n;
+  }
   return IB_OK;
 }
 

@@ -502,26 +502,56 @@ __global__ void k_f3_reduce(const cx<float>* __restrict__ part, cx<double>* __re
   }
 }
 
+// which fused engine runs a plan: 0 none, 1 the 3D 32^3 real-data form here, 2 collide_fused2
+static int fused_kind(const Plan* P) {
+  if ((P->precision == IB_PREC_FP32 && (P->dim == 2 || P->n == 16)) ||
+      (P->precision == IB_PREC_FP64 && P->dim == 2 && P->n == 32))
+    return 2;
+  return (P->dim == 3 && P->n == 32 && P->precision == IB_PREC_FP32) ? 1 : 0;
+}
+
+// per-cell scratch of the 3D 32^3 form: FP64 f-hat, accumulator and two work arrays; FP32
+// partial planes per radial node (twice with Kahan compensation), FP32 f-hat and f, the
+// Nyquist planes and the Nyquist term's partials
+static long long f3_sizes(const Plan* P, long long batch, ScratchSizes* z, size_t* part_bytes,
+                          size_t* fh32_bytes) {
+  using namespace f3;
+  const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
+  const int kp = P->kahan ? 2 : 1;
+  *part_bytes = (size_t)P->nr * kp * NPL * PLANE * sizeof(cx<float>);
+  *fh32_bytes = (size_t)17 * PLANE * sizeof(cx<float>) + (size_t)NPTS * sizeof(float);
+  const size_t aux_cell = *part_bytes + *fh32_bytes + nyq_scratch_bytes(P->nr);
+  const long long sub = fused_sub(P, batch, 256, 4 * b64 + aux_cell);
+  z->fhat = sub * b64;
+  z->acc = sub * b64;
+  z->work = 2 * sub * b64;
+  z->aux = sub * aux_cell;
+  return sub;
+}
+
+int fused_sizes(const Plan* P, long long batch, ScratchSizes* z) {
+  const int k = fused_kind(P);
+  if (k == 2) return fused2_sizes(P, batch, z);
+  if (k == 0) return 0;
+  size_t pb, fb;
+  f3_sizes(P, batch, z, &pb, &fb);
+  return 1;
+}
+
 int collide_fused(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                   int with_loss, double hb, hipStream_t st, int* handled) {
   using namespace f3;
   *handled = 0;
-  if ((P->precision == IB_PREC_FP32 && (P->dim == 2 || P->n == 16)) ||
-      (P->precision == IB_PREC_FP64 && P->dim == 2 && P->n == 32))
-    return collide_fused2(P, f, q, batch, pidx, with_loss, hb, st, handled);
-  if (!(P->dim == 3 && P->n == 32 && P->precision == IB_PREC_FP32)) return IB_OK;
+  const int kind = fused_kind(P);
+  if (kind == 2) return collide_fused2(P, f, q, batch, pidx, with_loss, hb, st, handled);
+  if (kind == 0) return IB_OK;
   *handled = 1;
-  const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 256);
-  const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
+  ScratchSizes z;
+  size_t part_bytes, fh32_bytes;
+  const long long sub = f3_sizes(P, batch, &z, &part_bytes, &fh32_bytes);
   const int kp = P->kahan ? 2 : 1;
-  const size_t part_bytes = (size_t)P->nr * kp * NPL * PLANE * sizeof(cx<float>);
-  const size_t fh32_bytes = (size_t)17 * PLANE * sizeof(cx<float>) + (size_t)NPTS * sizeof(float);
-  const size_t nyq_bytes = nyq_scratch_bytes(P->nr);
   int rc;
-  if ((rc = P->fhat.ensure(sub * b64))) return rc;
-  if ((rc = P->acc.ensure(sub * b64))) return rc;
-  if ((rc = P->work.ensure(2 * sub * b64))) return rc;
-  if ((rc = P->aux.ensure(sub * (part_bytes + fh32_bytes + nyq_bytes)))) return rc;
+  if ((rc = P->reserve(z))) return rc;
   cx<float>* part = (cx<float>*)P->aux.ptr;
   cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + sub * part_bytes);
   float* f32 = (float*)((char*)fh32 + sub * 17 * PLANE * sizeof(cx<float>));

@@ -408,18 +408,40 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
   }
 }
 
+// Scratch of the fused C2C forms per cell: FP64 f-hat, accumulator and two work arrays, then
+// the per-workgroup partial sums (one complex array per radial node, or per pair group), the
+// working-precision f-hat and f.
+template <typename T>
+static long long c2c_sizes(const Plan* P, long long batch, long long dflt, long long npts, long long parts,
+                           ScratchSizes* z) {
+  const size_t b64 = (size_t)npts * sizeof(cx<double>);
+  const size_t aux_cell = (size_t)(parts + 1) * npts * sizeof(cx<T>) + (size_t)npts * sizeof(T);
+  const long long sub = fused_sub(P, batch, dflt, 4 * b64 + aux_cell);
+  z->fhat = sub * b64;
+  z->acc = sub * b64;
+  z->work = 2 * sub * b64;
+  z->aux = sub * aux_cell;
+  return sub;
+}
+
+static long long fc3_sizes(const Plan* P, long long batch, ScratchSizes* z) {
+  return c2c_sizes<float>(P, batch, 1024, c3::NPTS, P->nr, z);
+}
+
+template <int N>
+static int f2_wpc(const Plan* P) {
+  return std::max(1, (P->nr * P->m + 16 * F2Shape<N>::G - 1) / (16 * F2Shape<N>::G));
+}
+
 static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                        int with_loss, double hb, hipStream_t st) {
   using namespace c3;
-  const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 1024);
-  const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
+  ScratchSizes z;
+  const long long sub = fc3_sizes(P, batch, &z);
   int rc;
-  if ((rc = P->fhat.ensure(sub * b64))) return rc;
-  if ((rc = P->acc.ensure(sub * b64))) return rc;
-  if ((rc = P->work.ensure(2 * sub * b64))) return rc;
+  if ((rc = P->reserve(z))) return rc;
   const size_t part_b = (size_t)sub * P->nr * NPTS * sizeof(cx<float>);
   const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<float>);
-  if ((rc = P->aux.ensure(part_b + fh_b + (size_t)sub * NPTS * sizeof(float)))) return rc;
   cx<float>* part = (cx<float>*)P->aux.ptr;
   cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + part_b);
   float* f32 = (float*)((char*)fh32 + fh_b);
@@ -660,15 +682,12 @@ template <typename T, int N>
 static int run_fusedw(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                       int with_loss, double hb, hipStream_t st) {
   constexpr long long NPTS = (long long)N * N;
-  const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 1024);
-  const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
+  ScratchSizes z;
+  const long long sub = c2c_sizes<T>(P, batch, 1024, NPTS, P->nr, &z);
   int rc;
-  if ((rc = P->fhat.ensure(sub * b64))) return rc;
-  if ((rc = P->acc.ensure(sub * b64))) return rc;
-  if ((rc = P->work.ensure(2 * sub * b64))) return rc;
+  if ((rc = P->reserve(z))) return rc;
   const size_t part_b = (size_t)sub * P->nr * NPTS * sizeof(cx<T>);
   const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<T>);
-  if ((rc = P->aux.ensure(part_b + fh_b + (size_t)sub * NPTS * sizeof(T)))) return rc;
   cx<T>* part = (cx<T>*)P->aux.ptr;
   cx<T>* fhn = (cx<T>*)((char*)P->aux.ptr + part_b);
   T* fT = (T*)((char*)fhn + fh_b);
@@ -720,19 +739,16 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
   constexpr int G = F2Shape<N>::G, THREADS = F2Shape<N>::THREADS;
   constexpr long long NPTS = (long long)N * N;
   const int npairs = P->nr * P->m;
-  const long long sub = std::min<long long>(batch, P->fused_cells > 0 ? P->fused_cells : 4096);
   // workgroups per cell fixed by the plan (16 rounds of G pairs each), not by the
   // batch size: a cell's FP32 summation order, and so its result, is the same in
   // any batch
-  const int wpc = std::max(1, (npairs + 16 * G - 1) / (16 * G));
-  const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
+  const int wpc = f2_wpc<N>(P);
+  ScratchSizes z;
+  const long long sub = c2c_sizes<float>(P, batch, 4096, NPTS, wpc, &z);
   int rc;
-  if ((rc = P->fhat.ensure(sub * b64))) return rc;
-  if ((rc = P->acc.ensure(sub * b64))) return rc;
-  if ((rc = P->work.ensure(2 * sub * b64))) return rc;
+  if ((rc = P->reserve(z))) return rc;
   const size_t part_b = (size_t)sub * wpc * NPTS * sizeof(cx<float>);
   const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<float>);
-  if ((rc = P->aux.ensure(part_b + fh_b + (size_t)sub * NPTS * sizeof(float)))) return rc;
   cx<float>* part = (cx<float>*)P->aux.ptr;
   cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + part_b);
   float* f32 = (float*)((char*)fh32 + fh_b);
@@ -776,6 +792,20 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
       return rc;
   }
   return IB_OK;
+}
+
+int fused2_sizes(const Plan* P, long long batch, ScratchSizes* z) {
+  if (P->precision == IB_PREC_FP64) {
+    if (P->dim == 2 && P->n == 32) return c2c_sizes<double>(P, batch, 1024, 32 * 32, P->nr, z), 1;
+    return 0;
+  }
+  if (P->dim == 3) return P->n == 16 ? (fc3_sizes(P, batch, z), 1) : 0;
+  switch (P->n) {
+    case 16: return c2c_sizes<float>(P, batch, 4096, 16 * 16, f2_wpc<16>(P), z), 1;
+    case 32: return c2c_sizes<float>(P, batch, 4096, 32 * 32, f2_wpc<32>(P), z), 1;
+    case 64: return c2c_sizes<float>(P, batch, 1024, 64 * 64, P->nr, z), 1;
+    default: return 0;
+  }
 }
 
 int collide_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,

@@ -474,17 +474,12 @@ static int launch_b(const BArgs<T>& B, hipStream_t st) {
 // chunk, 1,390 at 8, 1,470 at 16, 1,508 at 32, 1,391 at 64)
 constexpr long long kChunkPairs = 32;
 
-template <typename T, int N, int D>
-static int collide_staged(Plan* P, const double* f, double* q, long long batch, const int* pidx,
-                          int with_loss, double hb, int no_gain, hipStream_t st) {
-  using G = Geo<N, D>;
-  constexpr long long NPTS = G::NPTS;
-  const size_t arr = (size_t)NPTS * sizeof(cx<T>);
+// sub-batch of cells and pairs per chunk, bounded by the scratch budget.  Every
+// chunk re-reads and re-writes the sub-batch's accumulator, so the sub-batch is
+// cut until a chunk holds kChunkPairs pairs (3D 32^3 FP64: 64 cells x 32 pairs
+// per GiB instead of 1024 cells x 2 pairs)
+static void staged_shape(const Plan* P, long long batch, size_t arr, long long* bs_out, long long* pc_out) {
   const int npairs = P->nr * P->m;
-  // sub-batch of cells and pairs per chunk, bounded by the scratch budget.  Every
-  // chunk re-reads and re-writes the sub-batch's accumulator, so the sub-batch is
-  // cut until a chunk holds kChunkPairs pairs (3D 32^3 FP64: 64 cells x 32 pairs
-  // per GiB instead of 1024 cells x 2 pairs)
   long long bs = batch;
   const long long max_cells = (long long)std::max<size_t>(1, P->scratch_budget / (2 * arr));
   if (bs > max_cells) bs = max_cells;
@@ -499,6 +494,30 @@ static int collide_staged(Plan* P, const double* f, double* q, long long batch, 
   }
   if (pc > npairs) pc = npairs;
   if (pc < 2 && npairs >= 2) pc = 2;  // the epilogue needs 2 slots per cell
+  *bs_out = bs;
+  *pc_out = pc;
+}
+
+// pair-sum split of a chunk of np pairs over nb cells when the cells alone give fewer than
+// 512 workgroups (wg0): S partial accumulators, at most 512 MiB of them (and, in a
+// static-scratch call, what the region has left)
+static long long split_factor(long long wg0, long long np, long long nb, size_t arr) {
+  if (wg0 >= 512 || np < 4) return 1;
+  long long S = std::min<long long>((1024 + wg0 - 1) / wg0, np / 2);
+  S = std::min<long long>(S, std::max<long long>(1, (512LL << 20) / (nb * (long long)arr)));
+  S = std::min<long long>(S, (long long)std::min<size_t>(arena_left() / ((size_t)nb * arr), (size_t)S));
+  return S;
+}
+
+template <typename T, int N, int D>
+static int collide_staged(Plan* P, const double* f, double* q, long long batch, const int* pidx,
+                          int with_loss, double hb, int no_gain, hipStream_t st) {
+  using G = Geo<N, D>;
+  constexpr long long NPTS = G::NPTS;
+  const size_t arr = (size_t)NPTS * sizeof(cx<T>);
+  const int npairs = P->nr * P->m;
+  long long bs, pc;
+  staged_shape(P, batch, arr, &bs, &pc);
   IB_TRY(P->fhat.ensure(bs * arr));
   IB_TRY(P->acc.ensure(bs * arr));
   IB_TRY(P->work.ensure((size_t)bs * (size_t)std::max<long long>(pc, 2) * arr));
@@ -560,13 +579,7 @@ static int collide_staged(Plan* P, const double* f, double* q, long long batch, 
       A.first = (p0 == 0);
       // split the pair sum when the cells alone give fewer than 512 workgroups
       const long long wg0 = (long long)G::BLK * ((nb + G::SUBS - 1) / G::SUBS);
-      long long S = 1;
-      if (wg0 < 512 && np >= 4) {
-        S = std::min<long long>((1024 + wg0 - 1) / wg0, np / 2);
-        S = std::min<long long>(S, std::max<long long>(1, (512LL << 20) / ((long long)nb * arr)));
-        // a static-scratch call: the partials must fit in what the region has left
-        S = std::min<long long>(S, (long long)std::min<size_t>(arena_left() / ((size_t)nb * arr), (size_t)S));
-      }
+      const long long S = split_factor(wg0, np, nb, arr);
       if (S > 1) {
         IB_TRY(P->aux.ensure((size_t)(nb * S) * arr));
         A.part = (cx<T>*)P->aux.ptr;
@@ -614,6 +627,67 @@ static int collide_staged_dispatch(Plan* P, const double* f, double* q, long lon
       set_error("unsupported (dim, n) for the staged engine");
       return IB_ERR_UNSUPPORTED;
   }
+}
+
+template <int N, int D>
+static void staged_wg(long long nb, long long* wg0) {
+  using G = Geo<N, D>;
+  *wg0 = (long long)G::BLK * ((nb + G::SUBS - 1) / G::SUBS);
+}
+
+static long long staged_wg0(const Plan* P, long long nb) {
+  long long w = 0;
+  switch (P->dim * 100 + P->n) {
+    case 208: staged_wg<8, 2>(nb, &w); break;
+    case 216: staged_wg<16, 2>(nb, &w); break;
+    case 232: staged_wg<32, 2>(nb, &w); break;
+    case 264: staged_wg<64, 2>(nb, &w); break;
+    case 308: staged_wg<8, 3>(nb, &w); break;
+    case 316: staged_wg<16, 3>(nb, &w); break;
+    case 332: staged_wg<32, 3>(nb, &w); break;
+    default: w = 0;
+  }
+  return w;
+}
+
+// grouped chunk size: all cells of a plan in one batch
+static long long group_pc(const Plan* P, long long batch, size_t arr) {
+  long long pc = std::max<long long>(2, (long long)(P->scratch_budget / (batch * arr)));
+  return std::min<long long>(pc, std::max(P->nr * P->m, 2));
+}
+
+int staged_sizes(const Plan* P, long long batch, int grouped, ScratchSizes* z) {
+  if (batch <= 0) return IB_OK;
+  const size_t arr = (size_t)P->npts * (P->precision == IB_PREC_FP64 ? sizeof(cx<double>)
+                                                                     : sizeof(cx<float>));
+  const int npairs = P->nr * P->m;
+  if (grouped && !staged_runs_alone(P, batch)) {
+    const long long pc = group_pc(P, batch, arr);
+    z->fhat = batch * arr;
+    z->acc = batch * arr;
+    z->work = (size_t)batch * pc * arr;
+    // the grouped split depends on the other segments' workgroups: bound it by this
+    // segment alone
+    const long long S = split_factor(staged_wg0(P, batch), std::min<long long>(pc, npairs), batch, arr);
+    z->aux = S > 1 ? (size_t)(batch * S) * arr : 0;
+    return IB_OK;
+  }
+  long long bs, pc;
+  staged_shape(P, batch, arr, &bs, &pc);
+  z->fhat = bs * arr;
+  z->acc = bs * arr;
+  z->work = (size_t)bs * (size_t)std::max<long long>(pc, 2) * arr;
+  // the largest split over the sub-batch sizes (full and last) and chunk sizes (full and last)
+  size_t aux = 0;
+  const long long nbs[2] = {std::min(bs, batch), batch % bs ? batch % bs : bs};
+  const long long nps[2] = {std::min<long long>(pc, npairs), npairs % pc ? npairs % pc : pc};
+  for (long long nb : nbs)
+    for (long long np : nps) {
+      const long long S = split_factor(staged_wg0(P, nb), np, nb, arr);
+      if (S > 1) aux = std::max(aux, (size_t)(nb * S) * arr);
+    }
+  z->aux = aux;
+  return IB_OK;
 }
 
 // The grouped path keeps all cells of a plan in one batch, so its chunks hold
@@ -761,9 +835,8 @@ static int collide_group_t(Plan** P, int np, const double* const* f, double* con
   for (int i = 0; i < np; ++i) {
     if (batch[i] == 0) continue;
     const size_t arr = (size_t)P[i]->npts * sizeof(cx<T>);
-    long long pc = std::max<long long>(2, (long long)(P[i]->scratch_budget / (batch[i] * arr)));
+    const long long pc = group_pc(P[i], batch[i], arr);
     const int npairs = P[i]->nr * P[i]->m;
-    pc = std::min<long long>(pc, std::max(npairs, 2));
     IB_TRY(P[i]->fhat.ensure(batch[i] * arr));
     IB_TRY(P[i]->acc.ensure(batch[i] * arr));
     IB_TRY(P[i]->work.ensure((size_t)batch[i] * pc * arr));

@@ -39,6 +39,12 @@ struct Arena {
 extern thread_local Arena* g_arena;
 size_t arena_left();  // bytes left in the active arena; SIZE_MAX when none is active
 
+// Bytes a call asks of a plan's four scratch buffers.  Each engine computes its sizes with
+// the function it then allocates by, so ib_plan_scratch_need reports what a call will hold.
+struct ScratchSizes {
+  size_t fhat = 0, acc = 0, work = 0, aux = 0;
+};
+
 struct Plan {
   // geometry and quadrature (host copies)
   int dim = 0, n = 0, nr = 0, m = 0, precision = 0;
@@ -78,8 +84,30 @@ struct Plan {
   int timed_begin(hipStream_t st);
   void timed_end(hipStream_t st, long long cells);
   size_t scratch_budget = (size_t)1 << 30;
+  // ensure the four buffers hold z
+  int reserve(const ScratchSizes& z) {
+    int rc;
+    if ((rc = fhat.ensure(z.fhat)) || (rc = acc.ensure(z.acc)) || (rc = work.ensure(z.work)) ||
+        (rc = aux.ensure(z.aux)))
+      return rc;
+    return IB_OK;
+  }
 };
 
 int build_tables(Plan* p, hipStream_t s);
+
+// Cells per launch of a fused engine: the plan's fused_cells (0: the engine's default),
+// cut so that a sub-batch's scratch (per_cell bytes per cell) stays within the scratch
+// budget, and balanced over the launches the batch then needs.  A static-scratch call keeps
+// the uncut count: its region is the bound there.
+long long fused_sub(const Plan* P, long long batch, long long dflt, size_t per_cell);
+// Scratch sizes of the engine that runs a plan's batch of `batch` cells: the fused engines
+// (return 1), or 0 when the plan has no fused kernel.
+int fused_sizes(const Plan* P, long long batch, ScratchSizes* z);
+int fused2_sizes(const Plan* P, long long batch, ScratchSizes* z);
+// Upper bound of the staged engine's sizes, alone (grouped = 0) or as a segment of a grouped
+// launch (grouped = 1).
+int staged_sizes(const Plan* P, long long batch, int grouped, ScratchSizes* z);
+bool staged_runs_alone(const Plan* P, long long batch);
 
 }  // namespace ib

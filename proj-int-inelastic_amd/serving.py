@@ -10,7 +10,12 @@ collision operator of all problems of the class (all spatial cells of all
 problem with its own restitution e.  Between ticks, finished problems retire
 (their f is copied to the host, the slot and memory are freed) and arrived
 requests are admitted while the HBM budget allows; a request that does not fit
-waits in the queue (back-pressure) instead of failing.
+waits in the queue (back-pressure) instead of failing.  The budget counts each
+request's resident arrays (Request.nbytes) and each class's plan scratch, which
+the plan keeps between calls: admitting cells into a class reserves what the
+library reports the class's next call will hold (ib_plan_scratch_need).  Classes
+with no active problem give their plan (and its scratch) back when a request
+would not fit otherwise.
 
 Step of one tick, for a class with integrator "TPI2" and parameters
 (inner_steps, inner_dt, dt), is the reference's tel_proj_step2
@@ -19,6 +24,7 @@ tel_proj_step3 (:142-158), "PFE" proj_euler_step (:120-128), "Euler"
 euler_step (:82-84).  df of each
 problem is its TVD transport increment (1D jobs) plus dt Q(f)/kn.
 """
+import itertools
 import math
 import time
 from dataclasses import dataclass, field
@@ -74,7 +80,8 @@ class Request:
         return self.f0.shape[0] if self.xlen > 0 else 1
 
     def nbytes(self):
-        # resident f plus per-step temporaries (df, Q, transport) in FP64
+        # resident f plus per-step temporaries (df, Q, transport) in FP64; the class's plan
+        # scratch is reserved per class at admission (Server._admit)
         return 4 * self.f0.size * 8
 
 
@@ -102,11 +109,17 @@ class Result:
         return self.finished - self.arrival
 
 
+_SERIAL = itertools.count()
+
+
 class _Class:
     """Problems sharing one collision plan and one integrator."""
 
     def __init__(self, key, proto, autotune=False):
         t = dev.require_gpu()
+        # graphs are keyed by (serial, version): a class built after another was released
+        # never matches the released class's graph, whatever id() Python gives it
+        self.serial = next(_SERIAL)
         self.key = key
         self.proto = proto
         self.autotune = autotune
@@ -292,7 +305,7 @@ class _Group:
 
     def step(self):
         t = dev.torch()
-        gkey = tuple((id(c), c.version) for c in self.classes)
+        gkey = tuple((c.serial, c.version) for c in self.classes)
         if self.use_graph and self._graph is not None and self._graph_key == gkey:
             self._graph.replay()
             return
@@ -324,6 +337,9 @@ class Server:
         self.results = {}
         self.clock = clock
         self.t0 = None
+        self.reserved = {}   # class key -> plan scratch bytes counted in `used`
+        self.ticks = 0
+        self.peak_used = 0
 
     def submit(self, req):
         """Queue a request.  Requests that no state of the server could admit raise
@@ -354,6 +370,29 @@ class Server:
     def _now(self):
         return self.clock() - self.t0
 
+    def _class(self, key, proto):
+        cls = self.classes.get(key)
+        if cls is None:
+            cls = self.classes[key] = _Class(key, proto, self.autotune)
+            self.reserved[key] = 0
+        return cls
+
+    def _scratch_need(self, key, cells):
+        """Plan scratch the class will hold when it steps `cells` cells (never below what it
+        holds or has reserved: the plan's buffers only grow)."""
+        need, held = self.classes[key].coll.scratch_need(cells, grouped=True)
+        return max(need, held, self.reserved[key])
+
+    def _evict_idle(self, keep, admitted):
+        """Release the plans of classes with no active or newly admitted problem."""
+        freed = 0
+        for key in [k for k, c in self.classes.items()
+                    if k != keep and not c.active and k not in admitted]:
+            freed += self.reserved.pop(key, 0)
+            del self.classes[key]
+        self.used -= freed
+        return freed > 0
+
     def _admit(self, now):
         """Admit arrived requests in arrival order while memory allows."""
         admitted = {}
@@ -364,19 +403,29 @@ class Server:
                 waiting.append(r)
                 continue
             key = r.class_key()
+            cls = self._class(key, r)
             if key not in cells:
-                cls = self.classes.get(key)
-                cells[key] = sum(a.cells for a in cls.active) if cls else 0
-            fits_mem = self.used + r.nbytes() <= self.budget
+                cells[key] = sum(a.cells for a in cls.active)
             fits_cls = self.max_cells is None or cells[key] + r.cells <= self.max_cells
-            if not (fits_mem and fits_cls):
+            res = self._scratch_need(key, cells[key] + r.cells)
+            grow = r.nbytes() + res - self.reserved[key]
+            if fits_cls and self.used + grow > self.budget and self._evict_idle(key, admitted):
+                res = self._scratch_need(key, cells[key] + r.cells)
+                grow = r.nbytes() + res - self.reserved[key]
+            if not (fits_cls and self.used + grow <= self.budget):
+                busy = any(c.active for c in self.classes.values()) or admitted
+                if fits_cls and not busy:
+                    raise ValueError("request %r needs %d bytes of resident arrays and plan scratch, "
+                                     "above the HBM budget of %d" % (r.rid, self.used + grow, self.budget))
                 waiting.append(r)
                 continue
             cells[key] += r.cells
             n = int(round(r.t_final / r.dt))  # submit() checked that dt divides t_final
             a = _Active(req=r, start=0, cells=r.cells, steps_left=n, admitted_at=now)
-            admitted.setdefault(r.class_key(), []).append((a, r.f0))
-            self.used += r.nbytes()
+            admitted.setdefault(key, []).append((a, r.f0))
+            self.used += grow
+            self.reserved[key] = res
+            self.peak_used = max(self.peak_used, self.used)
         self.queue = waiting
         return admitted
 
@@ -390,8 +439,6 @@ class Server:
             now = self._now() if realtime else float("inf")
             new = self._admit(now)
             for key, items in new.items():
-                if key not in self.classes:
-                    self.classes[key] = _Class(key, items[0][0].req, self.autotune)
                 cls = self.classes[key]
                 cls.rebuild(cls.active, items)
             busy = [c for c in self.classes.values() if c.active]
@@ -403,6 +450,7 @@ class Server:
                 grp = self.groups.setdefault(gk, _Group(gk, self.use_graph))
                 grp.classes = [c for c in busy if c.proto.group_key() == gk]
                 grp.step()
+            self.ticks += 1
             for cls in busy:
                 for a in cls.active:
                     a.steps_left -= 1

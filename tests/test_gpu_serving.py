@@ -53,18 +53,60 @@ def test_alone_equals_mix(pkg, need_gpu):
         assert abs(alone.t - r.t_final) < 1e-12
 
 
+def _class_need(req, cells):
+    """Plan scratch a fresh operator of the request's class reports for `cells` cells."""
+    from proj_int_inelastic_amd.collision import FSInelasticVHSCollision, SpectralMesh
+    cfg = {"collision-model": {"dim": req.dim, "e": 1.0, "e0": 1.0, "gamma": req.gamma},
+           "velocity-mesh": {"nv": req.n, "s": req.s, "nr": req.nr}}
+    if req.dim == 2:
+        cfg["circle-quad-rule"] = {"nphi": req.m}
+    else:
+        cfg["spherical-design-rule"] = {"ssrule": "lebedev", "nsphr": req.m}
+    c = FSInelasticVHSCollision(cfg, SpectralMesh(cfg), precision=req.precision)
+    return c.scratch_need(cells, grouped=True)[0]
+
+
 def test_backpressure_queue(pkg, need_gpu):
+    """Room for two of four problems (resident arrays and the plan scratch of two cells):
+    the others wait, and every result equals the unconstrained server's."""
     from proj_int_inelastic_amd import serving
     L = 0.5 * (3 + math.sqrt(2)) * 3.0
     reqs = _requests(serving.Request, L)[:4]
-    budget = 2 * reqs[0].nbytes()
+    budget = 2 * reqs[0].nbytes() + _class_need(reqs[0], 2)
     srv = serving.Server(hbm_budget=budget)
     res = srv.run(reqs, realtime=False)
     assert sorted(res) == [0, 1, 2, 3]
     free = serving.Server().run(_requests(serving.Request, L)[:4], realtime=False)
     for k in res:
         assert np.abs(res[k].f - free[k].f).max() <= 1e-12 * np.abs(free[k].f).max()
-    assert srv.used == 0
+    assert srv.peak_used <= budget
+    assert srv.used == srv.reserved[reqs[0].class_key()]   # only the class's plan scratch is left
+
+
+def test_admission_counts_plan_scratch(pkg, need_gpu):
+    """Two 3D 32^3 FP32 problems whose resident arrays fit together but whose plan scratch for
+    two cells (fused partial planes) does not: the second queues until the first retires;
+    results equal the unconstrained run, and a problem that cannot fit alone is refused."""
+    import dataclasses
+    from proj_int_inelastic_amd import serving
+    L = 0.5 * (3 + math.sqrt(2)) * S3
+    reqs = [serving.Request(rid=i, f0=_gauss_nd(3, 32, L, 90 + i), t_final=0.25, e=0.5 + 0.3 * i, dim=3, n=32,
+                            nr=8, m=14, s=S3, gamma=1.0, precision="fp32", integrator="TPI2",
+                            inner_steps=(2, 2), inner_dt=(0.01, 0.05), dt=0.25) for i in range(2)]
+    one, two = _class_need(reqs[0], 1), _class_need(reqs[0], 2)
+    assert two > one > 2 * reqs[0].nbytes()
+    budget = 2 * reqs[0].nbytes() + (one + two) // 2
+    srv = serving.Server(hbm_budget=budget)
+    res = srv.run([dataclasses.replace(r) for r in reqs], realtime=False)
+    free_srv = serving.Server()
+    free = free_srv.run([dataclasses.replace(r) for r in reqs], realtime=False)
+    assert srv.ticks == 2 * free_srv.ticks            # one after the other
+    assert srv.peak_used <= budget
+    for k in free:
+        assert np.abs(res[k].f - free[k].f).max() <= 1e-12 * np.abs(free[k].f).max()
+    with pytest.raises(ValueError, match="above the HBM budget"):
+        serving.Server(hbm_budget=reqs[0].nbytes() + one // 2).run([dataclasses.replace(reqs[0])],
+                                                                   realtime=False)
 
 
 def test_graph_replay_and_mixed_shapes(pkg, need_gpu):
