@@ -528,14 +528,29 @@ struct CWArgs {
 // (E * fhat from L2, inverse along axis 1); B columns (inverse along axis 0, * f,
 // forward along axis 0; the column's two lanes are in one wave); C rows (forward
 // along axis 1, G * (.) and the sum).  Two workgroup barriers per pair.
+//
+// The work array is unpadded, element (r, c) at r N + (c ^ g(r)) with
+// g(r) = 2 floor(r / 2) ^ (3N/8) (r mod 2).  The lanes (p, h) of a wave access
+// (p, 2k + h) in phase A, (2k + h, p) at the end of phase B and (n, p), (p, n)
+// in the reads; with this permutation of each row every group of lanes served in
+// one LDS cycle touches distinct banks in all four patterns, whether the LDS
+// serves 128 or 256 bytes per cycle (a padded row of N + 1 elements puts lanes
+// (p, 1) and (p + 1, 0) on one bank in both write patterns).
+template <int N>
+__device__ __forceinline__ constexpr int wsw(int r) {
+  return (2 * (r >> 1)) ^ ((3 * N / 8) * (r & 1));
+}
+
 template <typename T, int N>
 __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
-  constexpr int H = N / 2, THREADS = 2 * N, PAD = N + 1;
+  constexpr int H = N / 2, THREADS = 2 * N, SWH = 3 * N / 8;
   constexpr long long NPTS = (long long)N * N;
-  __shared__ cx<T> W[N][PAD];
+  __shared__ cx<T> W[N * N];
   __shared__ cx<T> tE[2][2][N];
   __shared__ cx<T> tG[2][2][N];
-  __shared__ cx<T> twd[2][H];  // ones, then exp(2 pi i n / N)
+  // ones, then exp(2 pi i n / N); the second row starts one element later than H, so that
+  // the two lanes of a pencil (reading entry n of row 0 and of row 1 together) use different banks
+  __shared__ cx<T> twd[2][H + 1];
   const int tid = threadIdx.x;
   const long long id = blockIdx.x;
   const int c8 = (int)(id % 8);  // the 8 * nr workgroups of 8 cells share one XCD per cell
@@ -550,6 +565,16 @@ __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
   }
   const int p = tid >> 1, h = tid & 1;  // pencil and half
   const cx<T>* tw = twd[h];
+  // swizzled offsets: row p's element 2k + h is at p N + (2k ^ hg); row 2k + h's element p is
+  // at h N + 2k N + (2k ^ ph).  Each phase takes its offsets from an opaque copy, so the
+  // compiler forms the loop-invariant addresses next to their accesses instead of holding
+  // all of them in registers across the pair loop.
+  const int hg0 = h ^ wsw<N>(p);
+  const int ph0 = p ^ (SWH * h);
+  auto opaque = [](int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+  };
   const T sg = h ? (T)-1 : (T)1;
   const double aj = A.a[ps * A.nr + j] * A.rj[j];
   const double rpl = M_PI * A.rj[j] / A.L;
@@ -612,8 +637,9 @@ __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
       }
       half_dif<+1>(r, tw);
       const cx<T> ex = tE[b][0][p];
+      const int hg = opaque(hg0), rowp = opaque(p * N);
 #pragma unroll
-      for (int k = 0; k < H; ++k) W[p][2 * k + h] = ex * r[k];
+      for (int k = 0; k < H; ++k) W[rowp + ((2 * k) ^ hg)] = ex * r[k];
     }
     __syncthreads();
     // B: column p = y: inverse along axis 0 (kx -> x), * f, forward along axis 0;
@@ -621,8 +647,10 @@ __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
     // the previous pair, are behind the barrier above)
     {
       if (pre && mm + 1 < A.m) fetch(mm + 1);
+      int pc = opaque(p);
 #pragma unroll
-      for (int n = 0; n < H; ++n) split_row(r, W[n][p], W[n + H][p], n);
+      for (int n = 0; n < H; ++n)
+        split_row(r, W[n * N + (pc ^ wsw<N>(n))], W[(n + H) * N + (pc ^ wsw<N>(n + H))], n);
       half_dif<+1>(r, tw);
       // 32-bit offset made opaque per pair, so the addresses are formed next to
       // their loads instead of being held in registers across the pair loop
@@ -634,15 +662,20 @@ __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
         if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
       wave_sync2();  // both lanes of the column have read it
+      int ph = opaque(ph0), rowh = opaque(h * N);
 #pragma unroll
-      for (int k = 0; k < H; ++k) W[2 * k + h][p] = r[k];
+      for (int k = 0; k < H; ++k) W[rowh + 2 * k * N + ((2 * k) ^ ph)] = r[k];
       wave_sync2();  // ... and written their halves
+      pc = opaque(p);
 #pragma unroll
-      for (int n = 0; n < H; ++n) split_row(r, W[n][p], W[n + H][p], n);
+      for (int n = 0; n < H; ++n)
+        split_row(r, W[n * N + (pc ^ wsw<N>(n))], W[(n + H) * N + (pc ^ wsw<N>(n + H))], n);
       half_dif<-1>(r, tw);
       wave_sync2();
+      ph = opaque(ph0);
+      rowh = opaque(h * N);
 #pragma unroll
-      for (int k = 0; k < H; ++k) W[2 * k + h][p] = r[k];
+      for (int k = 0; k < H; ++k) W[rowh + 2 * k * N + ((2 * k) ^ ph)] = r[k];
     }
     if (mm + 1 < A.m) {
       if (pre) {
@@ -657,8 +690,9 @@ __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
     // row's own two lanes are the only readers, so the next pair's phase A may
     // overwrite it without a barrier)
     {
+      const int gp = opaque(wsw<N>(p)), rowp = opaque(p * N);
 #pragma unroll
-      for (int n = 0; n < H; ++n) split_row(r, W[p][n], W[p][n + H], n);
+      for (int n = 0; n < H; ++n) split_row(r, W[rowp + (n ^ gp)], W[rowp + ((n + H) ^ gp)], n);
       half_dif<-1>(r, tw);
       const cx<T> gx = (T)A.wsig[mm] * tG[b][0][p];
 #pragma unroll
