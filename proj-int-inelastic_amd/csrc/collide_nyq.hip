@@ -28,7 +28,7 @@
 // in registers for the whole kernel) and reduce it, angle by angle: over x in registers
 // (R1), over z across the 32 lanes of a half-wave (R3) and over y across the two
 // pencils, the half-waves and, through LDS, the waves (R2), by recursive halvings in a
-// lane-dependent register order that needs no selects (nyq_xl); wave q runs angle q's
+// lane-dependent register order that needs no selects (nyq_xl0); wave q runs angle q's
 // forward transforms (R1 + i R2 packed, R3); all threads weight the plane points with
 // separable per-angle tables and sum over angles in registers.  The per-(cell, j) plane
 // terms go to a buffer that k_f3n_reduce sums over j in FP64 into the full spectrum.
@@ -88,18 +88,35 @@ __device__ __forceinline__ float dpp_f(float v, int ctrl_id) {
   return __builtin_bit_cast(float, r);
 }
 
-// Register order of the x pencils.  Register i of lane l holds x = i ^ XL(l), with
-// XL(l) = 4 b2 ^ 12 b3 ^ 16 b4 ^ 16 b5 (b_k the lane's bits), a multiple of 4, so that
-// the reductions over lanes are plain recursive halvings with no lane-dependent
-// selects: a partner lane l ^ m holds, in register i + h, the x that lane l holds in
-// register i whenever XL(m) = h on the bits below 2h.  Over z (the 32 lanes of a
-// half-wave): l ^ 16 (h = 16, ds_bpermute), l ^ 15 (h = 8, DPP row_mirror), l ^ 7
-// (h = 4, DPP row_half_mirror), then a sum over the quad (l ^ 1, l ^ 2) of the four
-// remaining registers.  Over y (the two half-waves): l ^ 32 (h = 16).  Blocks of four
-// registers stay four consecutive x, so the pencils' P values load as 128-bit vectors.
-__device__ __forceinline__ int nyq_xl(int lane) {
-  return (((lane >> 2) & 1) * 4) ^ (((lane >> 3) & 1) * 12) ^ (((lane >> 4) & 1) * 16) ^
-         (((lane >> 5) & 1) * 16);
+// Register order of the x pencils.  Register i of lane l holds x = i ^ XL0(l), with
+// XL0(l) = 4 b2 ^ 12 b3 (b_k the lane's bits), a multiple of 4, so that the reductions over
+// lanes are recursive halvings with no lane-dependent selects.  Lanes l, l ^ 16 and l ^ 32
+// hold their x in the same registers, so the first halving of each reduction pairs register
+// i with register i + 16 of the partner row (or half-wave) through one v_permlane swap: after
+// swap(u[i], u[i + 16]) the sum u[i] + u[i + 16] is, in the even row, the pair's sum at
+// x = i ^ XL0 and, in the odd row, at x = (i + 16) ^ XL0, so the order of the 16 halved
+// registers becomes XL0 ^ 16 b4 (over z, rows l ^ 16) or XL0 ^ 16 b5 (over y, half-waves
+// l ^ 32).  The rest of the halving over z: l ^ 15 (h = 8, DPP row_mirror), l ^ 7 (h = 4, DPP
+// row_half_mirror), then a sum over the quad (l ^ 1, l ^ 2) of the four remaining registers.
+// Blocks of four registers stay four consecutive x, so the pencils' P values load as 128-bit
+// vectors.
+__device__ __forceinline__ int nyq_xl0(int lane) {
+  return (((lane >> 2) & 1) * 4) ^ (((lane >> 3) & 1) * 12);
+}
+
+// Rows z >= 16 of the P2 plane and of the R2 partials store element x at x ^ 16: lanes z and
+// z + 16 read the same x block, and their rows, 16 x 36 floats apart, fall on the same banks.
+__device__ __forceinline__ int nyq_rsw(int z) { return ((z >> 4) & 1) * 16; }
+
+// v_permlane16_swap (odd rows of a <-> even rows of b) and v_permlane32_swap (upper half-wave
+// of a <-> lower half-wave of b) with both registers written back; the compiler's builtins for
+// these return the first register for both results.  s_nop 1: the two wait states a VALU
+// write of an operand needs before the swap reads it.
+__device__ __forceinline__ void swap_rows16(float& a, float& b) {
+  asm volatile("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+}
+__device__ __forceinline__ void swap_halves32(float& a, float& b) {
+  asm volatile("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(a), "+v"(b));
 }
 
 // Orders a wave's LDS writes before its later reads of other lanes' data.
@@ -189,14 +206,15 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
 
   const int lane = tid & 63, wv = tid >> 6;
   const int z = tid & 31, yq = tid >> 5;  // N^3 roles: x pencils (y, z) and (y + 16, z)
-  const int xl = nyq_xl(lane);
+  const int xl0 = nyq_xl0(lane);                // order of the pencils' registers
+  const int xl1 = xl0 ^ (((lane >> 4) & 1) * 16);  // order after the halving over z
   const float sgz = (z & 1) ? -1.f : 1.f, sgy = (yq & 1) ? -1.f : 1.f;
 
   // f of the two pencils in the register order above, for the whole kernel
   float f0[N], f1[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    const int x = i ^ xl;
+    const int x = i ^ xl0;
     f0[i] = fc[x * PLANE + yq * N + z];
     f1[i] = fc[x * PLANE + (yq + 16) * N + z];
   }
@@ -253,7 +271,7 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
 #pragma unroll
         for (int r = 0; r < N; r += 4) {
           *(float4*)&P1T[a * RPS + r] = make_float4(v[r].re, v[r + 1].re, v[r + 2].re, v[r + 3].re);
-          *(float4*)&P2T[a * RPS + r] = make_float4(v[r].im, v[r + 1].im, v[r + 2].im, v[r + 3].im);
+          *(float4*)&P2T[a * RPS + (r ^ nyq_rsw(a))] = make_float4(v[r].im, v[r + 1].im, v[r + 2].im, v[r + 3].im);
         }
       } else {  // column a = y of P3
 #pragma unroll
@@ -276,7 +294,9 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
         // next to their loads each angle instead of occupying registers across the loop
         int tq = tid;
         asm volatile("" : "+v"(tq));
-        const int zz = tq & 31, y0 = tq >> 5, y1 = y0 + 16, xq = nyq_xl(tq & 63);
+        const int zz = tq & 31, y0 = tq >> 5, y1 = y0 + 16, xq = nyq_xl0(tq & 63);
+        const int xq2 = xq ^ (((tq >> 5) & 1) * 16);  // order after the halving over y
+        const int rsw = nyq_rsw(zz);
         // sigma_y sigma_z (s_x P1 - C sigma_y sigma_z) of the two pencils
         const float K1a = sgy * sgz * sx * P1T[zz * RPS + y0] - C;
         const float K1b = sgy * sgz * sx * P1T[zz * RPS + y1] - C;
@@ -285,7 +305,7 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
 #pragma unroll
         for (int k = 0; k < N / 4; ++k) {
           const int xb = (4 * k) ^ xq;  // first x of the register block 4k .. 4k + 3
-          const float4 p2 = *(const float4*)&P2T[zz * RPS + xb];
+          const float4 p2 = *(const float4*)&P2T[zz * RPS + (xb ^ rsw)];
           const float4 pa = *(const float4*)&P3T[y0 * RPS + xb];
           const float4 pb = *(const float4*)&P3T[y1 * RPS + xb];
           const float p2e[4] = {p2.x, p2.y, p2.z, p2.w};
@@ -307,17 +327,24 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
         // R2 over y: the two pencils in registers, the half-waves (l ^ 32), waves via LDS
         float t2[H];
 #pragma unroll
-        for (int i = 0; i < H; ++i)
-          t2[i] = (u0[i] + u1[i]) + __shfl_xor(u0[i + H] + u1[i + H], 32);
+        for (int i = 0; i < H; ++i) {
+          float a = u0[i] + u1[i], b = u0[i + H] + u1[i + H];
+          swap_halves32(a, b);
+          t2[i] = a + b;
+        }
 #pragma unroll
         for (int k = 0; k < H / 4; ++k)
-          *(float4*)&RP[wv][zz][(4 * k) ^ xq] = make_float4(t2[4 * k], t2[4 * k + 1], t2[4 * k + 2], t2[4 * k + 3]);
+          *(float4*)&RP[wv][zz][(4 * k) ^ xq2 ^ rsw] = make_float4(t2[4 * k], t2[4 * k + 1], t2[4 * k + 2], t2[4 * k + 3]);
         // R3 over z: recursive halving across the 32 lanes of the half-wave
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
           float* u = p ? u1 : u0;
 #pragma unroll
-          for (int i = 0; i < 16; ++i) u[i] += __shfl_xor(u[i + 16], 16);
+          for (int i = 0; i < 16; ++i) {
+            float a = u[i], b = u[i + 16];
+            swap_rows16(a, b);
+            u[i] = a + b;
+          }
 #pragma unroll
           for (int i = 0; i < 8; ++i) u[i] += dpp_f(u[i + 8], 0);
 #pragma unroll
@@ -327,7 +354,7 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) u[i] += dpp_f(u[i], 3);
 #pragma unroll
-          for (int i = 0; i < 4; ++i) r3[p][i] = sgy * u[i];  // R3(x = xl + i, y)
+          for (int i = 0; i < 4; ++i) r3[p][i] = sgy * u[i];  // R3(x = xl1 + i, y)
         }
       }
       __syncthreads();
@@ -340,12 +367,12 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
           const int y = yq + 16 * p;
           float s2 = 0.f;
 #pragma unroll
-          for (int w = 0; w < 8; ++w) s2 += RP[w][z][y];
+          for (int w = 0; w < 8; ++w) s2 += RP[w][z][y ^ nyq_rsw(z)];
           Za[y][z] = cx<float>{r1[p], sgz * s2};
-          // lane e of the quad writes x = xl + e (every lane of the quad holds all four)
+          // lane e of the quad writes x = xl1 + e (every lane of the quad holds all four)
           const int e = lane & 3;
           const float r3e = e == 0 ? r3[p][0] : e == 1 ? r3[p][1] : e == 2 ? r3[p][2] : r3[p][3];
-          Zb[xl + e][y] = cx<float>{r3e, 0.f};
+          Zb[xl1 + e][y] = cx<float>{r3e, 0.f};
         }
       }
       __syncthreads();
