@@ -82,6 +82,19 @@ int ib_plan_destroy(ib_plan* plan);
 int ib_plan_set_tuning(ib_plan* plan, long long chunk_pairs, long long fused_cells,
                        long long scratch_budget);
 
+/* Tile shape of the fused engines (new, like set_tuning; used by the autotuner).
+ * angle_split in [1, 64]: workgroups per (cell, radial node) of the forms that give one
+ * workgroup to a (cell, radial node j) and loop it over the M angles (FP32 3D 32^3 with its
+ * Nyquist-plane term, FP32 3D 16^3, FP32 2D 64^2, FP64 2D 32^2): split s takes the angles
+ * [s M / S, (s + 1) M / S) and writes its own partial spectrum, so a small batch fills more
+ * of the GPU (values above M act as M).  pair_rounds in [0, 1024]: rounds of quadrature
+ * pairs per workgroup of the FP32 2D 16^2 / 32^2 form, which sets its workgroups per cell
+ * (0 = 16).  Results change only by the order of the FP32 sums (the FP64 2D 32^2 form: of
+ * its FP64 sums); for a fixed tile shape a cell's result is the same in any batch.
+ * get_tiles returns the values in use. */
+int ib_plan_set_tiles(ib_plan* plan, int angle_split, int pair_rounds);
+int ib_plan_get_tiles(ib_plan* plan, int* angle_split, int* pair_rounds);
+
 /* Device scratch of a plan: *held = the bytes its buffers hold now; *need = the bytes they
  * will hold after a call of `cells` cells (ib_collide for grouped = 0, a segment of
  * ib_collide_group for grouped = 1) under the current tuning: exact for the fused engines,

@@ -49,6 +49,8 @@ SIGNATURES = {
     "ib_plan_set_params": (_I, [_P, _I, _c.POINTER(_D), _c.POINTER(_D), _P]),
     "ib_plan_destroy": (_I, [_P]),
     "ib_plan_set_tuning": (_I, [_P, _LL, _LL, _LL]),
+    "ib_plan_set_tiles": (_I, [_P, _I, _I]),
+    "ib_plan_get_tiles": (_I, [_P, _c.POINTER(_I), _c.POINTER(_I)]),
     "ib_plan_set_timing": (_I, [_P, _I]),
     "ib_plan_get_timing": (_I, [_P, _c.POINTER(_D), _c.POINTER(_LL), _c.POINTER(_LL)]),
     "ib_collide": (_I, [_P, _P, _P, _LL, _P, _I, _D, _P]),

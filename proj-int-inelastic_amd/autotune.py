@@ -2,10 +2,18 @@
 
 For a plan and a batch size it times the candidate settings of the engine the
 plan uses and keeps the fastest:
-  * fused forms (FP32 3D 32^3; FP32 2D 16^2, 32^2, 64^2 and 3D 16^3; FP64 2D
-    32^2): cells per fused launch (3D 32^3 64 .. 512, the others 256 .. 4096);
-  * staged form: the scratch budget, which sets the quadrature pairs per chunk
-    (items per pass launch) — 256 MiB .. 4 GiB.
+  * fused forms with one workgroup per (cell, radial node) (FP32 3D 32^3, 3D 16^3 and
+    2D 64^2; FP64 2D 32^2): cells per fused launch (3D 32^3 64 .. 512, the others
+    256 .. 4096) and the tile shape, i.e. how many workgroups share a (cell, radial
+    node)'s angles (1, 2, 4, 8; splits are tried while the launch has fewer than 2048
+    workgroups, so that small batches fill the GPU);
+  * the FP32 2D 16^2 / 32^2 form: cells per launch and rounds of quadrature pairs per
+    workgroup (4 .. 32), which set its workgroups per cell;
+  * staged form: the scratch budget (256 MiB .. 4 GiB) and the quadrature pairs per
+    chunk (automatic, 16, 64), which set the items per pass launch.
+A tile shape changes only the order of a cell's sums, so tuned and default results
+agree to the precision's rounding; for a fixed setting a cell's result is the same
+in any batch.
 Results are cached in memory and in a JSON file (``INELASTIC_TUNE_CACHE`` or
 ``~/.cache/inelastic_mi355x_tune.json``) keyed by (dim, n, nr, m, precision,
 batch bucket), so a process tunes each shape once.
@@ -53,20 +61,31 @@ def key_of(coll, batch):
                                         coll.precision, bucket(batch))
 
 
+def _splits(coll, batch):
+    """Angle splits worth timing: 1, and more while the launch stays under 2048 workgroups."""
+    r, _ = coll.sm.rquad()
+    return [s for s in (1, 2, 4, 8) if s == 1 or batch * len(r) * s // 2 < 2048]
+
+
 def candidates(coll, batch):
-    if coll.precision in ("fp32", "fp32k") and coll.ndim == 3 and coll.sm.nv == 32:
-        return [{"fused_cells": c} for c in (64, 128, 256, 512) if c <= max(64, batch)]
-    if coll.precision in ("fp32", "fp32k") and ((coll.ndim == 2 and coll.sm.nv in (16, 32, 64)) or
-                                                (coll.ndim == 3 and coll.sm.nv == 16)):
-        return [{"fused_cells": c} for c in (256, 1024, 4096) if c <= max(256, batch)]
-    if coll.precision == "fp64" and coll.ndim == 2 and coll.sm.nv == 32:
-        return [{"fused_cells": c} for c in (256, 1024, 4096) if c <= max(256, batch)]
-    return [{"scratch_budget": b << 20} for b in (256, 1024, 4096)]
+    d, n, prec = coll.ndim, coll.sm.nv, coll.precision
+    fp32 = prec in ("fp32", "fp32k")
+    if fp32 and d == 3 and n == 32:
+        return [{"fused_cells": c, "angle_split": s} for c in (64, 128, 256, 512) if c <= max(64, batch)
+                for s in _splits(coll, batch)]
+    if fp32 and d == 2 and n in (16, 32):
+        return [{"fused_cells": c, "pair_rounds": r} for c in (256, 1024, 4096) if c <= max(256, batch)
+                for r in (4, 8, 16, 32)]
+    if (fp32 and ((d == 2 and n == 64) or (d == 3 and n == 16))) or (prec == "fp64" and d == 2 and n == 32):
+        return [{"fused_cells": c, "angle_split": s} for c in (256, 1024, 4096) if c <= max(256, batch)
+                for s in _splits(coll, batch)]
+    return [{"scratch_budget": b << 20, "chunk_pairs": cp} for b in (256, 1024, 4096) for cp in (0, 16, 64)]
 
 
 def apply(coll, setting):
-    coll.set_tuning(chunk_pairs=0, fused_cells=setting.get("fused_cells", 0),
+    coll.set_tuning(chunk_pairs=setting.get("chunk_pairs", 0), fused_cells=setting.get("fused_cells", 0),
                     scratch_budget=setting.get("scratch_budget", 0))
+    coll.set_tiles(angle_split=setting.get("angle_split", 1), pair_rounds=setting.get("pair_rounds", 0))
 
 
 def tune(coll, f_dev, with_loss=True, reps=2, force=False):

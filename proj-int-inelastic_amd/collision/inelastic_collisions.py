@@ -122,6 +122,20 @@ class FSInelasticVHSCollision(BaseCollision):
         _lib.check(_lib.load().ib_plan_set_tuning(self._plan, int(chunk_pairs), int(fused_cells),
                                                   int(scratch_budget)), "ib_plan_set_tuning")
 
+    def set_tiles(self, angle_split=1, pair_rounds=0):
+        """Tile shape of the fused engines: workgroups per (cell, radial node), each over a
+        contiguous range of the angles (forms with one workgroup per (cell, j)), and rounds of
+        pairs per workgroup of the 2D 16^2 / 32^2 form (0 = 16)."""
+        _lib.check(_lib.load().ib_plan_set_tiles(self._plan, int(angle_split), int(pair_rounds)),
+                   "ib_plan_set_tiles")
+
+    def tiles(self):
+        """(angle_split, pair_rounds) in use."""
+        a, r = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.load().ib_plan_get_tiles(self._plan, ctypes.byref(a), ctypes.byref(r)),
+                   "ib_plan_get_tiles")
+        return a.value, r.value
+
     def scratch_need(self, cells, grouped=True):
         """(need, held): device bytes the plan's scratch will hold after a call of `cells`
         cells (a segment of collide_group when grouped) and the bytes it holds now."""

@@ -341,6 +341,26 @@ int ib_plan_set_tuning(ib_plan* p, long long chunk_pairs, long long fused_cells,
   return IB_OK;
 }
 
+int ib_plan_set_tiles(ib_plan* p, int angle_split, int pair_rounds) {
+  if (!p || angle_split < 1 || angle_split > 64 || pair_rounds < 0 || pair_rounds > 1024) {
+    set_error("invalid tile shape: angle_split in [1, 64], pair_rounds in [0, 1024]");
+    return IB_ERR_ARG;
+  }
+  p->angle_split = angle_split;
+  p->pair_rounds = pair_rounds;
+  return IB_OK;
+}
+
+int ib_plan_get_tiles(ib_plan* p, int* angle_split, int* pair_rounds) {
+  if (!p || !angle_split || !pair_rounds) {
+    set_error("null argument");
+    return IB_ERR_ARG;
+  }
+  *angle_split = p->splits();
+  *pair_rounds = p->pair_rounds;
+  return IB_OK;
+}
+
 int ib_plan_set_timing(ib_plan* p, int enable) {
   if (!p) {
     set_error("null plan");

@@ -42,7 +42,7 @@ int staged_epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void
 int collide_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                    int with_loss, double hb, hipStream_t st, int* handled);
 // exact anti-Hermitian (Nyquist-plane) term of the real-data form (collide_nyq.hip)
-size_t nyq_scratch_bytes(int nr);
+size_t nyq_scratch_bytes(int nparts);
 int launch_f3_nyq(Plan* P, const cx<float>* xp, const float* f32, cx<float>* out, cx<double>* acc64,
                   long long nb, const int* pidx, hipStream_t st);
 
@@ -69,6 +69,8 @@ struct F3Args {
   const double* a;
   const float* spg;
   int nr, m, nq;
+  int ns;  // workgroups per (cell, j), each over a contiguous range of the angles
+  int xg;  // 1: the workgroups of a cell on one XCD (batches of 8 cells or more)
   double L;
 };
 
@@ -160,11 +162,13 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   __shared__ cx<float> SP[N][PAD];
   const int tid = threadIdx.x;
   const long long id = blockIdx.x;
-  const int c8 = (int)(id % 8);
-  const int j = (int)((id / 8) % A.nr);
-  const long long cg = id / (8LL * A.nr);
-  const long long cell = cg * 8 + c8;
+  const int nrs = A.nr * A.ns;
+  long long cell;
+  int jj;  // partial index: radial node j, angle range asp
+  cell_of_block(id, nrs, A.xg, cell, jj);
+  const int j = jj / A.ns, asp = jj % A.ns;
   if (cell >= A.batch) return;  // uniform over the workgroup
+  const int m0 = (asp * A.m) / A.ns, mc = ((asp + 1) * A.m) / A.ns - m0;  // angles m0 .. m0 + mc - 1
   const int ps = A.pidx ? A.pidx[A.cell0 + cell] : 0;
   for (int q = tid; q < A.nq; q += THREADS) sp[q] = A.spg[((long long)ps * A.nr + j) * A.nq + q];
   const double aj = A.a[ps * A.nr + j] * A.rj[j];
@@ -183,7 +187,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   // (register-held, it spilled 444 bytes per lane and ran at 5.5k cell-evals/s;
   // FP32 in this buffer, 6.8k; bf16, 8.9k)
   unsigned* compb =
-      (unsigned*)(A.part + ((A.cell0 + cell) * A.nr + j) * KP * NPL * PLANE + NPL * PLANE);
+      (unsigned*)(A.part + ((A.cell0 + cell) * nrs + jj) * KP * NPL * PLANE + NPL * PLANE);
 
   cx<float> acc[N];
   cx<float> acc2[2], cmp2[2];
@@ -244,8 +248,8 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     }
   };
 
-  make_tables(0, 0);
-  if (A.m > 1) make_tables(1, 1);
+  make_tables(m0, 0);
+  if (mc > 1) make_tables(m0 + 1, 1);
   __syncthreads();
   make_x16(0);
   __syncthreads();
@@ -258,8 +262,8 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
   const cx<float>* fp = fh + (long long)p * PLANE + (long long)l * N;
 #pragma unroll
   for (int k = 0; k < N; ++k) v[k] = fp[k];
-  for (int mm = 0; mm < A.m; ++mm) {
-    const int b = mm & 3;
+  for (int t = 0; t < mc; ++t) {
+    const int mm = m0 + t, b = t & 3;
     const float wj = (float)(crad_j * A.wsig[mm]);
     const float ws_half = (float)(0.5 * A.wsig[mm]);
     // A: E * fhat, inverse along kz
@@ -351,9 +355,9 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     // side work: the packed plane of the previous pair, the phase tables two pairs
     // ahead and X16 of the next pair (tables are four-buffered, so none of these
     // touches a buffer another phase of this barrier interval reads)
-    if (mm > 0) packed_acc((mm - 1) & 3, wj_prev);
-    if (mm + 2 < A.m) make_tables(mm + 2, (mm + 2) & 3);
-    if (mm + 1 < A.m) make_x16((mm + 1) & 3);
+    if (t > 0) packed_acc((t - 1) & 3, wj_prev);
+    if (t + 2 < mc) make_tables(mm + 2, (t + 2) & 3);
+    if (t + 1 < mc) make_x16((t + 1) & 3);
     __syncthreads();
     // D: forward along ky
 #pragma unroll
@@ -378,7 +382,7 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     if constexpr (KAHAN) {  // loaded after the transform (fewer live registers)
       if (p != 0) {
 #pragma unroll
-        for (int k = 0; k < N; ++k) cv[k] = mm > 0 ? compb[k * THREADS + to] : 0u;
+        for (int k = 0; k < N; ++k) cv[k] = t > 0 ? compb[k * THREADS + to] : 0u;
       }
     }
     if (p == 0) {
@@ -407,9 +411,9 @@ __global__ __launch_bounds__(512, 1) void k_fused3(F3Args A) {
     wj_prev = wj;
   }
   __syncthreads();
-  packed_acc((A.m - 1) & 3, wj_prev);
+  packed_acc((mc - 1) & 3, wj_prev);
 
-  cx<float>* out = A.part + ((A.cell0 + cell) * A.nr + j) * KP * NPL * PLANE;
+  cx<float>* out = A.part + ((A.cell0 + cell) * nrs + jj) * KP * NPL * PLANE;
   cx<float> cfin[KAHAN ? N : 1];
   if constexpr (KAHAN) {
     if (p != 0) {
@@ -518,9 +522,10 @@ static long long f3_sizes(const Plan* P, long long batch, ScratchSizes* z, size_
   using namespace f3;
   const size_t b64 = (size_t)NPTS * sizeof(cx<double>);
   const int kp = P->kahan ? 2 : 1;
-  *part_bytes = (size_t)P->nr * kp * NPL * PLANE * sizeof(cx<float>);
+  const int parts = P->nr * P->splits();  // partial planes per cell: one per (j, angle range)
+  *part_bytes = (size_t)parts * kp * NPL * PLANE * sizeof(cx<float>);
   *fh32_bytes = (size_t)17 * PLANE * sizeof(cx<float>) + (size_t)NPTS * sizeof(float);
-  const size_t aux_cell = *part_bytes + *fh32_bytes + nyq_scratch_bytes(P->nr);
+  const size_t aux_cell = *part_bytes + *fh32_bytes + nyq_scratch_bytes(parts);
   const long long sub = fused_sub(P, batch, 256, 4 * b64 + aux_cell);
   z->fhat = sub * b64;
   z->acc = sub * b64;
@@ -556,7 +561,7 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
   cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + sub * part_bytes);
   float* f32 = (float*)((char*)fh32 + sub * 17 * PLANE * sizeof(cx<float>));
   cx<float>* xp = (cx<float>*)(f32 + sub * NPTS);          // sub x 3 planes
-  cx<float>* nyq_out = xp + sub * 3 * PLANE;               // sub x nr x 3 planes
+  cx<float>* nyq_out = xp + sub * 3 * PLANE;               // sub x nr x splits x 3 planes
   for (long long c0 = 0; c0 < batch; c0 += sub) {
     const long long nb = std::min(sub, batch - c0);
     const double* fc = f + c0 * NPTS;
@@ -581,20 +586,22 @@ int collide_fused(Plan* P, const double* f, double* q, long long batch, const in
     A.nr = P->nr;
     A.m = P->m;
     A.nq = P->nq;
+    A.ns = P->splits();
+    A.xg = nb >= 8;
     A.L = P->L;
     // pidx is indexed from the sub-batch start
     A.pidx = pidx ? pidx + c0 : nullptr;
-    const long long groups = (nb + 7) / 8;
     if ((rc = P->timed_begin(st))) return rc;
+    const unsigned grid = (unsigned)cell_grid(nb, P->nr * A.ns, A.xg);
     if (P->kahan)
-      hipLaunchKernelGGL(k_fused3<true>, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
+      hipLaunchKernelGGL(k_fused3<true>, dim3(grid), dim3(THREADS), 0, st, A);
     else
-      hipLaunchKernelGGL(k_fused3<false>, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
+      hipLaunchKernelGGL(k_fused3<false>, dim3(grid), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
     const long long half = nb * NPL * PLANE;
     hipLaunchKernelGGL(k_f3_reduce, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, st, part,
-                       (cx<double>*)P->acc.ptr, nb, P->nr, kp);
+                       (cx<double>*)P->acc.ptr, nb, P->nr * A.ns, kp);
     IB_HIP(hipGetLastError());
     if ((rc = launch_f3_nyq(P, xp, f32, nyq_out, (cx<double>*)P->acc.ptr, nb, A.pidx, st))) return rc;
     if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,

@@ -269,7 +269,7 @@ constexpr long long NPTS = (long long)N * N * N;
 struct C3Args {
   const cx<float>* fh;  // [cell][N][N][N], fhat / n
   const float* f32;     // [cell][N][N][N]
-  cx<float>* part;      // [cell][nr][N][N][N]
+  cx<float>* part;      // [cell][nr][ns][N][N][N]
   const int* pidx;
   long long batch;
   const double* sigma;
@@ -279,6 +279,8 @@ struct C3Args {
   const double* a;
   const float* spg;
   int nr, m, nq;
+  int ns;  // workgroups per (cell, j), each over a contiguous range of the angles
+  int xg;  // workgroup numbering (cell_of_block)
   double L;
 };
 
@@ -295,10 +297,13 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
   __shared__ float sp[3 * (N / 2) * (N / 2) + 1];
   const int tid = threadIdx.x;
   const long long id = blockIdx.x;
-  const int c8 = (int)(id % 8);  // the 8 * nr workgroups of 8 cells share one XCD per cell
-  const int j = (int)((id / 8) % A.nr);
-  const long long cell = (id / (8LL * A.nr)) * 8 + c8;
+  const int nrs = A.nr * A.ns;
+  long long cell;
+  int jj;  // partial index: radial node j, angle range asp
+  cell_of_block(id, nrs, A.xg, cell, jj);
+  const int j = jj / A.ns, asp = jj % A.ns;
   if (cell >= A.batch) return;  // uniform over the workgroup
+  const int m0 = (asp * A.m) / A.ns, mc = ((asp + 1) * A.m) / A.ns - m0;  // angles m0 .. m0 + mc - 1
   const int ps = A.pidx ? A.pidx[cell] : 0;
   for (int q = tid; q < A.nq; q += THREADS) sp[q] = A.spg[((long long)ps * A.nr + j) * A.nq + q];
   const double aj_pi = A.a[ps * A.nr + j] * A.rj[j] / M_PI;
@@ -334,11 +339,11 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
       (which ? tG : tE)[b][ax][k] = cx<float>{cs, sn};
     }
   };
-  make_tables(0, 0);
+  make_tables(m0, 0);
   __syncthreads();
   cx<float> v[N];
-  for (int mm = 0; mm < A.m; ++mm) {
-    const int b = mm & 1;
+  for (int t = 0; t < mc; ++t) {
+    const int mm = m0 + t, b = t & 1;
     // A: E * fhat along z at (kx, ky) = (r0, r1), inverse along z
     {
       const cx<float> row = tE[b][0][r0] * tE[b][1][r1];
@@ -373,7 +378,7 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
     for (int k = 0; k < N; ++k) W[k][r0][r1] = v[k];
     // the next pair's phase tables, into the other buffer: its last readers (phases A
     // and E of the previous pair) are behind the barrier above
-    if (mm + 1 < A.m) make_tables(mm + 1, b ^ 1);
+    if (t + 1 < mc) make_tables(mm + 1, b ^ 1);
     __syncthreads();
     // D: forward along y at (kx, z) = (r0, r1)
 #pragma unroll
@@ -400,7 +405,7 @@ __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
   // radial weight and sp(a_j r_j |k|), then this (cell, j)'s partial spectrum
   const float cj = (float)A.crad[ps * A.nr + j];
   const int kx = kfreq(r0, N), ky = kfreq(r1, N);
-  cx<float>* out = A.part + ((long long)cell * A.nr + j) * NPTS + (long long)(r0 * N + r1) * N;
+  cx<float>* out = A.part + ((long long)cell * nrs + jj) * NPTS + (long long)(r0 * N + r1) * N;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     const int kz = kfreq(k, N);
@@ -425,12 +430,14 @@ static long long c2c_sizes(const Plan* P, long long batch, long long dflt, long 
 }
 
 static long long fc3_sizes(const Plan* P, long long batch, ScratchSizes* z) {
-  return c2c_sizes<float>(P, batch, 1024, c3::NPTS, P->nr, z);
+  return c2c_sizes<float>(P, batch, 1024, c3::NPTS, (long long)P->nr * P->splits(), z);
 }
 
+// workgroups per cell of k_fused2: the plan's rounds of G pairs each (16 by default)
 template <int N>
 static int f2_wpc(const Plan* P) {
-  return std::max(1, (P->nr * P->m + 16 * F2Shape<N>::G - 1) / (16 * F2Shape<N>::G));
+  const int per = (P->pair_rounds > 0 ? P->pair_rounds : 16) * F2Shape<N>::G;
+  return std::max(1, (P->nr * P->m + per - 1) / per);
 }
 
 static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, const int* pidx,
@@ -440,7 +447,8 @@ static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, con
   const long long sub = fc3_sizes(P, batch, &z);
   int rc;
   if ((rc = P->reserve(z))) return rc;
-  const size_t part_b = (size_t)sub * P->nr * NPTS * sizeof(cx<float>);
+  const int ns = P->splits();
+  const size_t part_b = (size_t)sub * P->nr * ns * NPTS * sizeof(cx<float>);
   const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<float>);
   cx<float>* part = (cx<float>*)P->aux.ptr;
   cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + part_b);
@@ -469,14 +477,15 @@ static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, con
     A.nr = P->nr;
     A.m = P->m;
     A.nq = P->nq;
+    A.ns = ns;
+    A.xg = nb >= 8;
     A.L = P->L;
-    const long long groups = (nb + 7) / 8;
     if ((rc = P->timed_begin(st))) return rc;
-    hipLaunchKernelGGL(k_fusedc3, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
+    hipLaunchKernelGGL(k_fusedc3, dim3((unsigned)cell_grid(nb, P->nr * ns, A.xg)), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
     hipLaunchKernelGGL(k_f2_reduce<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
-                       (cx<double>*)P->acc.ptr, nb, P->nr, NPTS);
+                       (cx<double>*)P->acc.ptr, nb, P->nr * ns, NPTS);
     IB_HIP(hipGetLastError());
     if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
                                    with_loss, hb, pc)))
@@ -507,7 +516,7 @@ struct CWArgs {
   const T* f;           // [cell][N][N]
   const cx<T>* ephase;  // [pair][2][N] or null
   const cx<T>* gphase;  // [param][pair][2][N] or null
-  cx<T>* part;          // [cell][nr][N][N]
+  cx<T>* part;          // [cell][nr][ns][N][N]
   const int* pidx;
   long long batch;
   const double* sigma;
@@ -517,6 +526,8 @@ struct CWArgs {
   const double* a;
   const T* spg;
   int nr, m, nq;
+  int ns;  // workgroups per (cell, j), each over a contiguous range of the angles
+  int xg;  // workgroup numbering (cell_of_block)
   double L;
 };
 
@@ -541,7 +552,9 @@ __device__ __forceinline__ constexpr int wsw(int r) {
   return (2 * (r >> 1)) ^ ((3 * N / 8) * (r & 1));
 }
 
-template <typename T, int N>
+// SPLIT = false is the one-range form (ns = 1): its FP64 instance keeps two waves per SIMD,
+// which the split form's extra scalar values would cost it.
+template <typename T, int N, bool SPLIT>
 __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
   constexpr int H = N / 2, THREADS = 2 * N, SWH = 3 * N / 8;
   constexpr long long NPTS = (long long)N * N;
@@ -553,10 +566,15 @@ __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
   __shared__ cx<T> twd[2][H + 1];
   const int tid = threadIdx.x;
   const long long id = blockIdx.x;
-  const int c8 = (int)(id % 8);  // the 8 * nr workgroups of 8 cells share one XCD per cell
-  const int j = (int)((id / 8) % A.nr);
-  const long long cell = (id / (8LL * A.nr)) * 8 + c8;
+  const int ns = SPLIT ? A.ns : 1;
+  const int nrs = A.nr * ns;
+  long long cell;
+  int jj;  // partial index: radial node j, angle range asp
+  cell_of_block(id, nrs, SPLIT ? A.xg : 1, cell, jj);
+  const int j = jj / ns, asp = jj % ns;
   if (cell >= A.batch) return;  // uniform over the workgroup
+  const int m0 = SPLIT ? (asp * A.m) / ns : 0;  // angles m0 .. mend - 1
+  const int mend = SPLIT ? ((asp + 1) * A.m) / ns : A.m;
   const int ps = A.pidx ? A.pidx[cell] : 0;
   if (tid < N) {
     const int n = tid % H;
@@ -621,11 +639,11 @@ __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
   cx<T> acc[H];
 #pragma unroll
   for (int k = 0; k < H; ++k) acc[k] = cx<T>{(T)0, (T)0};
-  load_tables(0, 0);
+  load_tables(m0, 0);
   __syncthreads();
   cx<T> r[H];
-  for (int mm = 0; mm < A.m; ++mm) {
-    const int b = mm & 1;
+  for (int mm = m0; mm < mend; ++mm) {
+    const int b = (mm - m0) & 1;
     // A: row p = kx: E * fhat, inverse along axis 1 (ky -> y); the row factor
     // E_x(kx) is constant along the row and applied to the outputs
     {
@@ -646,7 +664,7 @@ __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
     // the next pair's phase tables (their buffer's last readers, phases A and C of
     // the previous pair, are behind the barrier above)
     {
-      if (pre && mm + 1 < A.m) fetch(mm + 1);
+      if (pre && mm + 1 < mend) fetch(mm + 1);
       int pc = opaque(p);
 #pragma unroll
       for (int n = 0; n < H; ++n)
@@ -677,7 +695,7 @@ __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
 #pragma unroll
       for (int k = 0; k < H; ++k) W[rowh + 2 * k * N + ((2 * k) ^ ph)] = r[k];
     }
-    if (mm + 1 < A.m) {
+    if (mm + 1 < mend) {
       if (pre) {
         tE[b ^ 1][tax][tk] = nt[0];
         tG[b ^ 1][tax][tk] = nt[1];
@@ -700,11 +718,11 @@ __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
       wave_sync2();
     }
   }
-  // radial weight and sp(a_j r_j |k|), then this (cell, j)'s partial spectrum
+  // radial weight and sp(a_j r_j |k|), then this (cell, j, angle range)'s partial spectrum
   const T cj = (T)A.crad[ps * A.nr + j];
   const T* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
   const int kx = kfreq(p, N);
-  cx<T>* out = A.part + ((long long)cell * A.nr + j) * NPTS + (long long)p * N;
+  cx<T>* out = A.part + ((long long)cell * nrs + jj) * NPTS + (long long)p * N;
 #pragma unroll
   for (int k = 0; k < H; ++k) {
     const int ky = kfreq(2 * k + h, N);
@@ -717,10 +735,11 @@ static int run_fusedw(Plan* P, const double* f, double* q, long long batch, cons
                       int with_loss, double hb, hipStream_t st) {
   constexpr long long NPTS = (long long)N * N;
   ScratchSizes z;
-  const long long sub = c2c_sizes<T>(P, batch, 1024, NPTS, P->nr, &z);
+  const int ns = P->splits();
+  const long long sub = c2c_sizes<T>(P, batch, 1024, NPTS, (long long)P->nr * ns, &z);
   int rc;
   if ((rc = P->reserve(z))) return rc;
-  const size_t part_b = (size_t)sub * P->nr * NPTS * sizeof(cx<T>);
+  const size_t part_b = (size_t)sub * P->nr * ns * NPTS * sizeof(cx<T>);
   const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<T>);
   cx<T>* part = (cx<T>*)P->aux.ptr;
   cx<T>* fhn = (cx<T>*)((char*)P->aux.ptr + part_b);
@@ -751,14 +770,21 @@ static int run_fusedw(Plan* P, const double* f, double* q, long long batch, cons
     A.nr = P->nr;
     A.m = P->m;
     A.nq = P->nq;
+    A.ns = ns;
     A.L = P->L;
-    const long long groups = (nb + 7) / 8;
+    // the one-range form keeps the XCD-grouped numbering (without a split, each of a small
+    // batch's workgroups has a CU to itself either way)
+    A.xg = ns > 1 ? nb >= 8 : 1;
     if ((rc = P->timed_begin(st))) return rc;
-    hipLaunchKernelGGL((k_fusedw<T, N>), dim3((unsigned)(groups * 8 * P->nr)), dim3(2 * N), 0, st, A);
+    const unsigned grid = (unsigned)cell_grid(nb, P->nr * ns, A.xg);
+    if (ns > 1)
+      hipLaunchKernelGGL((k_fusedw<T, N, true>), dim3(grid), dim3(2 * N), 0, st, A);
+    else
+      hipLaunchKernelGGL((k_fusedw<T, N, false>), dim3(grid), dim3(2 * N), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
     hipLaunchKernelGGL(k_f2_reduce<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
-                       (cx<double>*)P->acc.ptr, nb, P->nr, NPTS);
+                       (cx<double>*)P->acc.ptr, nb, P->nr * ns, NPTS);
     IB_HIP(hipGetLastError());
     if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
                                    with_loss, hb, pc)))
@@ -830,14 +856,15 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
 
 int fused2_sizes(const Plan* P, long long batch, ScratchSizes* z) {
   if (P->precision == IB_PREC_FP64) {
-    if (P->dim == 2 && P->n == 32) return c2c_sizes<double>(P, batch, 1024, 32 * 32, P->nr, z), 1;
+    if (P->dim == 2 && P->n == 32)
+      return c2c_sizes<double>(P, batch, 1024, 32 * 32, (long long)P->nr * P->splits(), z), 1;
     return 0;
   }
   if (P->dim == 3) return P->n == 16 ? (fc3_sizes(P, batch, z), 1) : 0;
   switch (P->n) {
     case 16: return c2c_sizes<float>(P, batch, 4096, 16 * 16, f2_wpc<16>(P), z), 1;
     case 32: return c2c_sizes<float>(P, batch, 4096, 32 * 32, f2_wpc<32>(P), z), 1;
-    case 64: return c2c_sizes<float>(P, batch, 1024, 64 * 64, P->nr, z), 1;
+    case 64: return c2c_sizes<float>(P, batch, 1024, 64 * 64, (long long)P->nr * P->splits(), z), 1;
     default: return 0;
   }
 }

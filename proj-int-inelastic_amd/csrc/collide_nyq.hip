@@ -61,6 +61,8 @@ struct F3NArgs {
   const double* a;
   const float* spg;
   int nr, m, nq;
+  int ns;  // workgroups per (cell, j), each over a contiguous range of the angles
+  int xg;  // workgroup numbering (cell_of_block)
   double L;
 };
 
@@ -157,10 +159,13 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
   __shared__ __align__(16) float RP[8][N][RPS];
   const int tid = threadIdx.x;
   const long long id = blockIdx.x;
-  const int c8 = (int)(id % 8);
-  const int j = (int)((id / 8) % A.nr);
-  const long long cell = (id / (8LL * A.nr)) * 8 + c8;
+  const int nrs = A.nr * A.ns;
+  long long cell;
+  int jj;  // partial index: radial node j, angle range asp
+  cell_of_block(id, nrs, A.xg, cell, jj);
+  const int j = jj / A.ns, asp = jj % A.ns;
   if (cell >= A.batch) return;  // uniform over the workgroup
+  const int m0 = (asp * A.m) / A.ns, mend = ((asp + 1) * A.m) / A.ns;  // angles m0 .. mend - 1
   const int ps = A.pidx ? A.pidx[cell] : 0;
   const double aj = A.a[ps * A.nr + j] * A.rj[j];
   const double rpl_pi = A.rj[j] / A.L;
@@ -176,7 +181,7 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
     const int q = t / (6 * N), r = t % (6 * N);
     const int which = r / (3 * N), ax = (r % (3 * N)) / N, k = r % N;
     const int mm = g0 + q;
-    if (mm < A.m) {
+    if (mm < mend) {
       const double sg = A.sigma[mm * 3 + ax];
       const double kf = (double)kfreq(k, N);
       double tt = which == 0 ? -(rpl_pi * sg * kf) : (aj_pi * sg * kf);
@@ -190,7 +195,7 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
   auto make_weights = [&](int g0, int bb, int t) {
     const int q = t / (5 * N), r = t % (5 * N), tab = r / N, k = r % N;
     const int mm = g0 + q;
-    if (mm < A.m) {
+    if (mm < mend) {
       const float ws = (float)A.wsig[mm];
       const cx<float>* g = &tG[bb][q][0][0];
       const cx<float> gx = g[H], gy = g[N + H], gz = g[2 * N + H];
@@ -223,11 +228,11 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
 #pragma unroll
   for (int i = 0; i < PER; ++i) acc[i] = cx<float>{0.f, 0.f};
 
-  for (int t = tid; t < 6 * G * N; t += THREADS) make_tables(0, 0, t);
+  for (int t = tid; t < 6 * G * N; t += THREADS) make_tables(m0, 0, t);
   __syncthreads();
-  for (int t = tid; t < 5 * G * N; t += THREADS) make_weights(0, 0, t);
-  for (int g0 = 0, bb = 0; g0 < A.m; g0 += G, bb ^= 1) {
-    const int ng = A.m - g0 < G ? A.m - g0 : G;
+  for (int t = tid; t < 5 * G * N; t += THREADS) make_weights(m0, 0, t);
+  for (int g0 = m0, bb = 0; g0 < mend; g0 += G, bb ^= 1) {
+    const int ng = mend - g0 < G ? mend - g0 : G;
     // A (wave q < ng): P1 + i P2 = ifft2(h_z (h_y X1 + i h_x X2)), P3 = ifft2(h_x h_y X3),
     // left as the real planes P1T[z][y], P2T[z][x], P3T[y][x]
     if (wv < ng) {
@@ -393,11 +398,11 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
       fft_reg<N, -1>(v);
 #pragma unroll
       for (int r = 0; r < N; ++r) Zt[r][a] = v[r];
-    } else if (wv >= G && g0 + G < A.m) {
+    } else if (wv >= G && g0 + G < mend) {
       for (int t = tid - G * 64; t < 6 * G * N; t += THREADS - G * 64) make_tables(g0 + G, bb ^ 1, t);
     }
     __syncthreads();
-    if (g0 + G < A.m)
+    if (g0 + G < mend)
       for (int t = tid; t < 5 * G * N; t += THREADS) make_weights(g0 + G, bb ^ 1, t);
     // E (all): V on the plane points times the separable weights, summed over angles
 #pragma unroll
@@ -426,7 +431,7 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
   }
   const float cj = (float)A.crad[ps * A.nr + j];
   const float* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
-  cx<float>* out = A.out + (cell * A.nr + j) * SLOTS;
+  cx<float>* out = A.out + (cell * nrs + jj) * SLOTS;
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
     const int s = tid + i * THREADS;
@@ -470,12 +475,13 @@ __global__ void k_f3n_reduce(const cx<float>* __restrict__ part, cx<double>* __r
   *o = cx<double>{o->re + re, o->im + im};
 }
 
-size_t nyq_scratch_bytes(int nr) {
-  return (size_t)(nr + 1) * f3n::SLOTS * sizeof(cx<float>);  // out planes per j + packed fhat planes
+size_t nyq_scratch_bytes(int nparts) {
+  // out planes per (j, angle range) + packed fhat planes
+  return (size_t)(nparts + 1) * f3n::SLOTS * sizeof(cx<float>);
 }
 
 // Launch the Nyquist term for nb cells whose FP32 f (f32) and fhat/n planes (xp) are packed;
-// out holds nr * SLOTS complex FP32 per cell; acc64 receives the FP64 sum.
+// out holds nr * splits * SLOTS complex FP32 per cell; acc64 receives the FP64 sum.
 int launch_f3_nyq(Plan* P, const cx<float>* xp, const float* f32, cx<float>* out, cx<double>* acc64,
                   long long nb, const int* pidx, hipStream_t st) {
   using namespace f3n;
@@ -494,13 +500,14 @@ int launch_f3_nyq(Plan* P, const cx<float>* xp, const float* f32, cx<float>* out
   A.nr = P->nr;
   A.m = P->m;
   A.nq = P->nq;
+  A.ns = P->splits();
+  A.xg = nb >= 8;
   A.L = P->L;
-  const long long groups = (nb + 7) / 8;
-  hipLaunchKernelGGL(k_f3_nyq, dim3((unsigned)(groups * 8 * P->nr)), dim3(THREADS), 0, st, A);
+  hipLaunchKernelGGL(k_f3_nyq, dim3((unsigned)cell_grid(nb, P->nr * A.ns, A.xg)), dim3(THREADS), 0, st, A);
   IB_HIP(hipGetLastError());
   const long long tot = nb * SLOTS;
   hipLaunchKernelGGL(k_f3n_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, out, acc64,
-                     nb, P->nr);
+                     nb, P->nr * A.ns);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }

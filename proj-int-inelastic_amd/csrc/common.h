@@ -195,4 +195,24 @@ __host__ __device__ inline double sp_func(int dim, double x) {
   return dim == 2 ? j0_midpoint(x) : sinc_x(x);
 }
 
+// Workgroup id -> (cell, partial index jj) with nrs workgroups per cell.  xg = 1: the nrs
+// workgroups of cell c are numbered i = c (mod 8), so they run on one XCD (the dispatcher
+// gives workgroup i to XCD i mod 8) and share the cell's fhat and f in its L2; the grid has
+// ceil(batch / 8) * 8 * nrs workgroups.  xg = 0 (batches under 8 cells): consecutive
+// numbering, so a small batch's workgroups spread over all XCDs; grid batch * nrs.
+__device__ __forceinline__ void cell_of_block(long long id, int nrs, int xg, long long& cell, int& jj) {
+  if (xg) {
+    jj = (int)((id / 8) % nrs);
+    cell = (id / (8LL * nrs)) * 8 + id % 8;
+  } else {
+    jj = (int)(id % nrs);
+    cell = id / nrs;
+  }
+}
+
+// grid of a per-(cell, partial) launch, numbered as cell_of_block reads it
+static inline long long cell_grid(long long batch, int nrs, int xg) {
+  return xg ? (batch + 7) / 8 * 8 * nrs : batch * nrs;
+}
+
 }  // namespace ib

@@ -76,6 +76,13 @@ struct Plan {
   DevBuf fhat, acc, work, aux;
   long long chunk_pairs = 0;  // pairs per staged chunk (0 = automatic)
   long long fused_cells = 0;  // cells per fused launch (0 = automatic)
+  // tile shape of the fused engines: workgroups per (cell, radial node) of the forms with one
+  // workgroup per (cell, j), each taking a contiguous range of the angles (1 = the whole loop);
+  // rounds of pairs per workgroup of the 2D 16^2 / 32^2 form (0 = 16)
+  int angle_split = 1;
+  int pair_rounds = 0;
+  // splits actually used: at most one per angle
+  int splits() const { return angle_split < 1 ? 1 : (angle_split > m ? m : angle_split); }
   hipStream_t stream = nullptr;  // stream of the current call
   // optional HIP-event timing of the dominant kernel (k_fused3 / staged slab pass)
   int timing = 0;

@@ -123,7 +123,6 @@ class _Class:
         self.key = key
         self.proto = proto
         self.autotune = autotune
-        self._tuned = set()
         d = proto.dim
         cfg = {"collision-model": {"dim": d, "e": 1.0, "e0": 1.0, "gamma": proto.gamma},
                "velocity-mesh": {"nv": proto.n, "s": proto.s, "nr": proto.nr}}
@@ -168,13 +167,11 @@ class _Class:
             self._tune()
 
     def _tune(self):
-        """Pick the launch shape for this class's cell count (once per size bucket;
-        the autotuner's cache makes later servers of this process reuse it)."""
+        """Apply the launch and tile shape of this class's cell-count bucket: timed the first
+        time the process meets the bucket, then taken from the autotuner's cache (a class
+        whose cell count moves between buckets gets each bucket's own setting back)."""
         from . import autotune as at
-        b = at.bucket(self.f.shape[0])
-        if b not in self._tuned:
-            self._tuned.add(b)
-            at.tune(self.coll, self.f, with_loss=True)
+        at.tune(self.coll, self.f, with_loss=True)
 
     def df_from_q(self, f, q, h):
         """Stacked df of every problem: TVD transport (1D jobs) + h Q / kn.

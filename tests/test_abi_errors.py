@@ -96,6 +96,19 @@ def test_collision_entry_points_reject(pkg):
     assert lib.ib_plan_destroy(None) == 0
 
 
+@pytest.mark.parametrize("split,rounds", [(0, 0), (65, 0), (1, -1), (1, 1025)])
+def test_tile_shape_rejects(pkg, split, rounds):
+    """Tile shapes outside [1, 64] splits and [0, 1024] rounds are refused before the plan
+    is touched (the fake plan pointer is never dereferenced)."""
+    lib = pkg._lib.load()
+    assert lib.ib_plan_set_tiles(FAKE, split, rounds) == -1
+    assert b"tile shape" in lib.ib_last_error()
+    assert lib.ib_plan_set_tiles(None, 1, 0) == -1
+    a, r = ctypes.c_int(), ctypes.c_int()
+    assert lib.ib_plan_get_tiles(None, ctypes.byref(a), ctypes.byref(r)) == -1
+    assert lib.ib_plan_get_tiles(FAKE, None, ctypes.byref(r)) == -1
+
+
 def test_status_to_exception_mapping(pkg):
     L = pkg._lib
     L.check(L.IB_OK)

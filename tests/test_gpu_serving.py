@@ -126,7 +126,9 @@ def test_graph_replay_and_mixed_shapes(pkg, need_gpu):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_autotuned_server_matches_default(pkg, need_gpu, tmp_path, monkeypatch, precision):
-    """A server that tunes each class's launch shape gives the default server's results."""
+    """A server that tunes each class's launch and tile shape gives the default server's
+    results: to FP64 round-off, and within the FP32 bar in FP32 (a tile shape changes the
+    order of the FP32 angular sums)."""
     import dataclasses
     from proj_int_inelastic_amd import autotune, serving
     monkeypatch.setenv("INELASTIC_TUNE_CACHE", str(tmp_path / "tune.json"))
@@ -138,7 +140,7 @@ def test_autotuned_server_matches_default(pkg, need_gpu, tmp_path, monkeypatch, 
     assert len(autotune._MEM) >= 2
     for k in b:
         err = np.abs(a[k].f - b[k].f).max() / np.abs(b[k].f).max()
-        assert err <= 1e-12, (k, err)
+        assert err <= (1e-12 if precision == "fp64" else 1e-5), (k, err)
 
 
 def test_realtime_arrivals_match_offline(pkg, need_gpu):
