@@ -619,14 +619,18 @@ def main():
     ap.add_argument("--check-cells", type=int, default=2)
     ap.add_argument("--cpu-cells", type=int, default=36)
     ap.add_argument("--mode", choices=["bench", "profile", "profile2d", "profile2d32fp64", "profile2d64",
-                                       "profile3d16", "copy"],
+                                       "profile3d16", "uneven", "copy"],
                     default="bench",
                     help="profile: FP32 form only on --cells cells (for rocprofv3); "
                          "profile2d: the 2D 32^2 FP32 form only on --cells-2d cells; "
                          "profile2d32fp64: the 2D 32^2 FP64 form only on --cells-2d cells; "
                          "profile2d64: the 2D 64^2 FP32 form only on --cells-2d64 cells; "
                          "profile3d16: the 3D 16^3 FP32 form only on --cells-3d16 cells; "
+                         "uneven: the FP32 uneven batch only (--uneven-problems, default shapes, "
+                         "no graph replay; --per-plan-fused for one fused launch per plan); "
                          "copy: a plain 1 GiB device copy (PMC byte-counter calibration)")
+    ap.add_argument("--per-plan-fused", action="store_true",
+                    help="uneven mode: one fused launch per plan instead of the grouped launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-kahan", action="store_true", help="skip the fp32k side measurement")
@@ -685,6 +689,20 @@ def main():
     if args.mode == "profile2d64":
         print(json.dumps({"mode": "profile2d64", **run_shape("2d64", args.cells_2d64, args.steps,
                                                              args.warmup, ("fp32",))}))
+        return
+    if args.mode == "uneven":
+        from proj_int_inelastic_amd import serving
+        from proj_int_inelastic_amd.collision.inelastic_collisions import set_group_mode
+        set_group_mode(not args.per_plan_fused)
+        reqs = uneven_requests(args.uneven_problems, "fp32")
+        srv = serving.Server(hbm_budget=200 << 30, use_graph=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = srv.run(reqs, realtime=False)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        print(json.dumps({"mode": "uneven", "grouped_fused": not args.per_plan_fused, "solves": len(res),
+                          "seconds": dt, "solves_per_s": len(res) / dt}))
         return
     if args.mode == "profile3d16":
         print(json.dumps({"mode": "profile3d16", **run_shape("3d16", args.cells_3d16, args.steps,
@@ -825,7 +843,16 @@ def main():
         u32, r32 = uneven_run("fp32", args.uneven_problems, autotune=not args.no_uneven_autotune)
         _progress("uneven batch FP32")
         out["uneven_batch"] = {"workload": "homogeneous cooling, 2D 16^2/32^2/64^2 + 3D 16^3/32^3, "
-                                           "TPI2 to t=2, grouped launches", "fp32": u32}
+                                           "TPI2 to t=2, grouped launches (one launch of the fused "
+                                           "pair loops of all FP32 shapes per df stage)", "fp32": u32}
+        from proj_int_inelastic_amd.collision.inelastic_collisions import set_group_mode
+        set_group_mode(False)
+        try:
+            u32p, _ = uneven_run("fp32", args.uneven_problems, autotune=not args.no_uneven_autotune)
+        finally:
+            set_group_mode(True)
+        out["uneven_batch"]["fp32_one_fused_launch_per_plan"] = u32p
+        _progress("uneven batch FP32, one fused launch per plan")
         if not args.no_fp64:
             u64, r64 = uneven_run("fp64", args.uneven_problems)
             out["uneven_batch"]["fp64"] = u64

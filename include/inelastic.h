@@ -137,17 +137,29 @@ int ib_collide_static(ib_plan* plan, const double* f, double* q, long long batch
 
 /* Grouped form of ib_collide for an uneven batch: nplans plans (any grid shapes,
  * one precision), each with its own cells f[i], q[i], batch[i], param_idx[i]
- * (param_idx may be NULL).  Every stage of the staged engine runs as ONE persistent
- * kernel launch over the work of all plans (up to 8 plans per launch); plans
- * whose shape has a fused kernel (FP32 3D 32^3 and 16^3, 2D 16^2, 32^2 and 64^2;
- * FP64 2D 32^2) run it, and a plan whose batch is too large for chunks of 32
- * quadrature pairs within its scratch budget runs alone in sub-batches.  A plan may
+ * (param_idx may be NULL).  FP32 plans with a fused kernel (3D 32^3 and 16^3, 2D 16^2,
+ * 32^2 and 64^2) run their quadrature-pair loops as ONE grouped kernel launch (up to 8
+ * plans per launch; see ib_set_group_mode); every stage of the staged engine runs as ONE
+ * persistent kernel launch over the work of the staged plans (up to 8 per launch); the
+ * FP64 2D 32^2 fused kernel runs per plan, and a plan whose batch is too large for its
+ * scratch budget (fused) or for chunks of 32 quadrature pairs (staged) runs alone in
+ * sub-batches.  A plan may
  * appear only once (its scratch and parameter-set tables serve one segment): a
  * repeated plan is IB_ERR_ARG before any work is queued.  New: the reference
  * evaluates one operator per call (src/collision/base.py:32-58). */
 int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, double* const* q,
                      const long long* batch, const int* const* param_idx, int with_loss,
                      double heat_bath, void* stream);
+
+/* Mode of ib_collide_group for FP32 plans with a fused form (3D 32^3 and 16^3, 2D 16^2, 32^2
+ * and 64^2): 1 (default) runs the quadrature-pair loops of all of them as ONE grouped kernel
+ * launch per call (plans whose whole batch fits their scratch budget; the others run alone),
+ * with each plan's FP64 prologue, Nyquist-plane term, FP64 reduce and epilogue around it; 0
+ * runs one fused launch per plan.  Both modes give bit-identical results.  Process-wide; set it
+ * between calls, not while another thread is inside ib_collide_group.  New: the grouped launch
+ * of an uneven batch (the reference evaluates one operator per call,
+ * src/collision/base.py:32-58). */
+int ib_set_group_mode(int fused_grouped);
 
 /* Laplacian of f (spectral), out = Re ifft(-pi^2 |k|^2 / L^2 fft f).
  * Replaces src/collision/inelastic_collisions.py:229-230. */

@@ -57,6 +57,7 @@ SIGNATURES = {
     "ib_collide_static": (_I, [_P, _P, _P, _LL, _P, _I, _D, _P]),
     "ib_collide_group": (_I, [_I, _c.POINTER(_P), _c.POINTER(_P), _c.POINTER(_P), _c.POINTER(_LL),
                               _c.POINTER(_P), _I, _D, _P]),
+    "ib_set_group_mode": (_I, [_I]),
     "ib_laplacian": (_I, [_P, _P, _P, _LL, _P]),
     "ib_moments": (_I, [_P, _P, _LL, _I, _I, _D, _P]),
     "ib_project_conservative": (_I, [_P, _LL, _I, _I, _D, _P]),

@@ -254,8 +254,16 @@ class FSInelasticVHSCollision(BaseCollision):
         return out if is_dev else out.cpu().numpy()
 
 
+def set_group_mode(fused_grouped=True):
+    """collide_group's handling of FP32 operators with a fused form: one grouped launch of
+    all their quadrature-pair loops (True, the default) or one fused launch per operator
+    (False); both give bit-identical results.  Process-wide."""
+    _lib.check(_lib.load().ib_set_group_mode(int(bool(fused_grouped))), "ib_set_group_mode")
+
+
 def collide_group(colls, fs, with_loss=False, heat_bath=None, es=None, gammas=None):
-    """Q(f, f) of an uneven batch in grouped launches (one launch per stage).
+    """Q(f, f) of an uneven batch in grouped launches (one launch per stage: the pair loops
+    of all FP32 fused-form operators share one launch, the staged operators' passes another).
 
     colls: operators of one precision (any grid shapes); fs: device tensors
     (*B_i, N_i^d_i); es/gammas: optional per-cell arrays per operator.

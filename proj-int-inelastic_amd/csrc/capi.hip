@@ -8,6 +8,7 @@
 #include <string>
 
 #include "common.h"
+#include "group.h"
 #include "plan.h"
 
 namespace ib {
@@ -102,6 +103,13 @@ int launch_phase_tables2(Plan* P, hipStream_t st);
 int collide_group(Plan** P, int np, const double* const* f, double* const* q,
                   const long long* batch, const int* const* pidx, int with_loss, double hb,
                   hipStream_t st);
+// grouped launch of the fused pair loops (group.h, collide_fused.hip, collide_group.hip)
+int fused_group_front(Plan* P, const double* f, long long batch, const int* pidx, hipStream_t st,
+                      GroupSeg* seg, int* ok, int* which);
+int fused_group_back(Plan* P, int which, const double* f, double* q, long long batch, const int* pidx,
+                     int with_loss, double hb, hipStream_t st);
+// 1: ib_collide_group runs the fused pair loops of its FP32 plans as one grouped launch
+static int g_group_fused = 1;
 int launch_tvd1d(const double* f, double* df, long long outer, int nx, long long nv,
                  const double* v1, long long vmod, double dtdx, int order, int limiter, int bc_lo,
                  int bc_hi, int accumulate, hipStream_t st);
@@ -496,11 +504,30 @@ int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, 
         return IB_ERR_ARG;
       }
   }
+  // FP32 plans with a fused form whose whole batch fits their scratch budget: their pair
+  // loops run as segments of one grouped launch (group.h) between their front and back work
+  std::vector<GroupSeg> segs;
+  struct Grouped {
+    Plan* p;
+    int i, which;
+  };
+  std::vector<Grouped> grouped;
   for (int i = 0; i < nplans; ++i) {
     ib_plan* p = plans[i];
     if (batch[i] == 0) continue;
     p->stream = st;
     const int* pi = param_idx ? param_idx[i] : nullptr;
+    if (g_group_fused && prec != IB_PREC_FP64) {
+      GroupSeg seg{};
+      int ok = 0, which = 0;
+      int rc = fused_group_front(p, f[i], batch[i], pi, st, &seg, &ok, &which);
+      if (rc) return rc;
+      if (ok) {
+        segs.push_back(seg);
+        grouped.push_back(Grouped{p, i, which});
+        continue;
+      }
+    }
     {  // shapes with a fused kernel run it
       int handled = 0;
       int rc = collide_fused(p, f[i], q[i], batch[i], pi, with_loss, heat_bath, st, &handled);
@@ -524,6 +551,25 @@ int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, 
                            gi.data() + s0, with_loss, heat_bath, st);
     if (rc) return rc;
   }
+  for (size_t s0 = 0; s0 < segs.size(); s0 += kMaxGroupSegs) {
+    const int n = (int)std::min<size_t>(kMaxGroupSegs, segs.size() - s0);
+    int rc = launch_fused_group(segs.data() + s0, n, st);
+    if (rc) return rc;
+  }
+  for (const Grouped& g : grouped) {
+    const int* pi = param_idx ? param_idx[g.i] : nullptr;
+    int rc = fused_group_back(g.p, g.which, f[g.i], q[g.i], batch[g.i], pi, with_loss, heat_bath, st);
+    if (rc) return rc;
+  }
+  return IB_OK;
+}
+
+int ib_set_group_mode(int fused_grouped) {
+  if (fused_grouped != 0 && fused_grouped != 1) {
+    set_error("group mode must be 0 or 1");
+    return IB_ERR_ARG;
+  }
+  g_group_fused = fused_grouped;
   return IB_OK;
 }
 

@@ -29,6 +29,7 @@
 #include <algorithm>
 
 #include "common.h"
+#include "group.h"
 #include "plan.h"
 
 namespace ib {
@@ -37,146 +38,15 @@ int staged_prologue_fp64(Plan* P, const double* f, long long nb, void* fhat64, v
 int staged_epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void* acc64,
                          void* fhat64, void* work, int with_loss, double hb, const int* pidx);
 
-struct F2Args {
-  const cx<float>* fh;  // [cell][N][N], fhat / n
-  const cx<float>* ephase;  // [pair][2][N] or null
-  const cx<float>* gphase;  // [param][pair][2][N] or null
-  const float* f32;     // [cell][N][N]
-  cx<float>* part;      // [cell][wpc][N][N]
-  const int* pidx;
-  long long batch;
-  int wpc;
-  const double* sigma;
-  const double* rj;
-  const double* wsig;
-  const double* crad;
-  const double* a;
-  const float* spg;
-  int nr, m, nq;
-  double L;
-};
-
-template <int N>
-struct F2Shape {
-  static constexpr int THREADS = N >= 32 ? 128 : 256;
-  static constexpr int G = THREADS / N;
-};
-
 template <int N>
 __global__ __launch_bounds__(F2Shape<N>::THREADS) void k_fused2(F2Args A) {
-  constexpr int THREADS = F2Shape<N>::THREADS, G = F2Shape<N>::G, PAD = N + 1;
-  // 16^2 stages the cell's fhat / n and f in LDS; 32^2 reads them from L2, as
-  // staging would cost the fourth workgroup per CU (8% slower measured; at 16^2
-  // the LDS staging is 12% faster and does not change the occupancy)
-  constexpr bool STAGE = N < 32;
-  constexpr int SN = STAGE ? N : 1;
-  __shared__ cx<float> Wg[G][N][PAD];
-  __shared__ cx<float> tab[G][4][N];  // E axis 0, E axis 1, G axis 0, G axis 1
-  __shared__ cx<float> fhs[SN][SN + 1];
-  __shared__ float fs[SN][SN + 1];
-  const int tid = threadIdx.x;
-  const int g = tid / N, l = tid % N;
+  __shared__ typename F2L<N>::Wg Wg;
+  __shared__ typename F2L<N>::Tab tab;
+  __shared__ typename F2L<N>::Fhs fhs;
+  __shared__ typename F2L<N>::Fs fs;
   const long long cell = blockIdx.x / A.wpc;
-  const int w = (int)(blockIdx.x % A.wpc);
   if (cell >= A.batch) return;  // uniform over the workgroup
-  const int ps = A.pidx ? A.pidx[cell] : 0;
-  const cx<float>* fhr = A.fh + cell * N * N + (long long)l * N;  // row l of fhat / n
-  const float* fcl = A.f32 + cell * N * N + l;                     // column l of f
-  if constexpr (STAGE) {
-    for (int e = tid; e < N * N; e += THREADS) {
-      fhs[e / N][e % N] = A.fh[cell * N * N + e];
-      fs[e / N][e % N] = A.f32[cell * N * N + e];
-    }
-  }
-  const int npairs = A.nr * A.m;
-  const int stride = A.wpc * G;
-  const int rounds = npairs > w * G ? (npairs - w * G + stride - 1) / stride : 0;
-  const int kl = kfreq(l, N);
-  cx<float> acc[N];
-#pragma unroll
-  for (int k = 0; k < N; ++k) acc[k] = cx<float>{0.f, 0.f};
-  __syncthreads();
-  cx<float> v[N];
-  for (int r = 0; r < rounds; ++r) {
-    const int pair = w * G + g + r * stride;
-    const bool act = pair < npairs;
-    const int j = act ? pair / A.m : 0, mm = act ? pair % A.m : 0;
-    if (act) {
-      if (A.ephase) {
-        const cx<float>* e = A.ephase + (long long)pair * 2 * N;
-        tab[g][0][l] = e[l];
-        tab[g][1][l] = e[N + l];
-      } else {
-        const double rj = A.rj[j];
-#pragma unroll
-        for (int ax = 0; ax < 2; ++ax) {
-          double sn, cs;
-          sincos(-(M_PI * rj * A.sigma[mm * 2 + ax] * (double)kl) / A.L, &sn, &cs);
-          tab[g][ax][l] = cx<float>{(float)cs, (float)sn};
-        }
-      }
-      if (A.gphase) {
-        const cx<float>* gt = A.gphase + ((long long)ps * npairs + pair) * 2 * N;
-        tab[g][2][l] = gt[l];
-        tab[g][3][l] = gt[N + l];
-      } else {
-        const double aj = A.a[ps * A.nr + j] * A.rj[j];
-#pragma unroll
-        for (int ax = 0; ax < 2; ++ax) {
-          double sn, cs;
-          sincos(aj * A.sigma[mm * 2 + ax] * (double)kl, &sn, &cs);
-          tab[g][2 + ax][l] = cx<float>{(float)cs, (float)sn};
-        }
-      }
-    }
-    __syncthreads();
-    if (act) {  // A
-      const cx<float> e0 = tab[g][0][l];
-#pragma unroll
-      for (int k = 0; k < N; ++k)
-        v[k] = (e0 * tab[g][1][k]) * (STAGE ? fhs[l % SN][k % SN] : fhr[k]);
-      fft_reg<N, +1>(v);
-#pragma unroll
-      for (int k = 0; k < N; ++k) Wg[g][l][k] = v[k];
-    }
-    __syncthreads();
-    if (act) {  // B
-#pragma unroll
-      for (int i = 0; i < N; ++i) v[i] = Wg[g][i][l];
-      fft_reg<N, +1>(v);
-#pragma unroll
-      for (int i = 0; i < N; ++i) v[i] = (STAGE ? fs[i % SN][l % SN] : fcl[i * N]) * v[i];
-      fft_reg<N, -1>(v);
-#pragma unroll
-      for (int i = 0; i < N; ++i) Wg[g][i][l] = v[i];
-    }
-    __syncthreads();
-    if (act) {  // C
-#pragma unroll
-      for (int k = 0; k < N; ++k) v[k] = Wg[g][l][k];
-      fft_reg<N, -1>(v);
-      const float wj = (float)(A.crad[ps * A.nr + j] * A.wsig[mm]);
-      const float* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
-      const cx<float> g0 = tab[g][2][l];
-#pragma unroll
-      for (int k = 0; k < N; ++k) {
-        const int kk = kfreq(k, N);
-        const cx<float> Gk = (wj * sp[kl * kl + kk * kk]) * (g0 * tab[g][3][k]);
-        acc[k] = acc[k] + Gk * v[k];
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int k = 0; k < N; ++k) Wg[g][l][k] = acc[k];
-  __syncthreads();
-  cx<float>* out = A.part + (cell * A.wpc + w) * N * N;
-  for (int e = tid; e < N * N; e += THREADS) {
-    cx<float> s{0.f, 0.f};
-#pragma unroll
-    for (int gg = 0; gg < G; ++gg) s = s + Wg[gg][e / N][e % N];
-    out[e] = s;
-  }
+  fused2_body<N>(A, cell, (int)(blockIdx.x % A.wpc), threadIdx.x, Wg, tab, fhs, fs);
 }
 
 // Phase tables of every quadrature pair (j, m), axis a and frequency index k:
@@ -250,182 +120,42 @@ __global__ void k_f2_reduce(const cx<T>* __restrict__ part, cx<double>* __restri
   acc64[gid] = cx<double>{re, im};
 }
 
-// Orders a wave's LDS accesses around an exchange between its own lanes: a wave's
-// LDS instructions execute in program order, so only the compiler must be kept
-// from moving them.
-__device__ __forceinline__ void wave_sync2() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-namespace c3 {
-constexpr int N = 16;
-constexpr int PAD = N + 1;
-constexpr int THREADS = N * N;
-constexpr long long NPTS = (long long)N * N * N;
-}  // namespace c3
-
-struct C3Args {
-  const cx<float>* fh;  // [cell][N][N][N], fhat / n
-  const float* f32;     // [cell][N][N][N]
-  cx<float>* part;      // [cell][nr][ns][N][N][N]
-  const int* pidx;
-  long long batch;
-  const double* sigma;
-  const double* rj;
-  const double* wsig;
-  const double* crad;
-  const double* a;
-  const float* spg;
-  int nr, m, nq;
-  int ns;  // workgroups per (cell, j), each over a contiguous range of the angles
-  int xg;  // workgroup numbering (cell_of_block)
-  double L;
-};
-
-// Pair schedule (2 workgroup barriers per quadrature pair): plane kx of the work
-// array is written and read in passes A, B, D and E only by the 16 threads (kx, *),
-// which share a wave, so those exchanges need no barrier; the x pass C reads and
-// writes every plane.  W[x][y][z] rows are padded to N + 1, so every pass reads
-// and writes 16 lanes of distinct banks.
 __global__ __launch_bounds__(c3::THREADS) void k_fusedc3(C3Args A) {
   using namespace c3;
   __shared__ cx<float> W[N][N][PAD];
   __shared__ cx<float> tE[2][3][N];
   __shared__ cx<float> tG[2][3][N];
   __shared__ float sp[3 * (N / 2) * (N / 2) + 1];
-  const int tid = threadIdx.x;
-  const long long id = blockIdx.x;
-  const int nrs = A.nr * A.ns;
   long long cell;
-  int jj;  // partial index: radial node j, angle range asp
-  cell_of_block(id, nrs, A.xg, cell, jj);
-  const int j = jj / A.ns, asp = jj % A.ns;
+  int jj;
+  cell_of_block(blockIdx.x, A.nr * A.ns, A.xg, cell, jj);
   if (cell >= A.batch) return;  // uniform over the workgroup
-  const int m0 = (asp * A.m) / A.ns, mc = ((asp + 1) * A.m) / A.ns - m0;  // angles m0 .. m0 + mc - 1
-  const int ps = A.pidx ? A.pidx[cell] : 0;
-  for (int q = tid; q < A.nq; q += THREADS) sp[q] = A.spg[((long long)ps * A.nr + j) * A.nq + q];
-  const double aj_pi = A.a[ps * A.nr + j] * A.rj[j] / M_PI;
-  const double rpl_pi = A.rj[j] / A.L;  // pi r_j / L in half turns
-  // pencil role: the two fixed indices (r0, r1) of the thread's pencil in every pass
-  const int r0 = tid / N, r1 = tid % N;
-  cx<float> fhv[N];  // fhat along z at (kx, ky) = (r0, r1)
-  float fv[N];       // f along x at (y, z) = (r0, r1)
-  {
-    const cx<float>* fh = A.fh + cell * NPTS + (long long)(r0 * N + r1) * N;
-    const float* fc = A.f32 + cell * NPTS + r0 * N + r1;
-#pragma unroll
-    for (int k = 0; k < N; ++k) fhv[k] = fh[k];
-#pragma unroll
-    for (int x = 0; x < N; ++x) fv[x] = fc[x * N * N];
-  }
-  cx<float> acc[N];
-#pragma unroll
-  for (int k = 0; k < N; ++k) acc[k] = cx<float>{0.f, 0.f};
-  // phase factors of pair mm along each axis (the separable factors of
-  // inelastic_collisions.py:97-105), FP32 sincospi of the FP64-reduced angle
-  auto make_tables = [&](int mm, int b) {
-    if (tid < 6 * N) {
-      const int which = tid / (3 * N), ax = (tid % (3 * N)) / N, k = tid % N;
-      const double s = A.sigma[mm * 3 + ax];
-      const double kf = (double)kfreq(k, N);
-      // the angle in half turns, reduced to [-1, 1] in FP64; the FP32 sincospi of the
-      // reduced value is as accurate as the FP32 rounding of an FP64 sincos
-      double t = which == 0 ? -(rpl_pi * s * kf) : (aj_pi * s * kf);
-      t -= 2.0 * rint(0.5 * t);
-      float sn, cs;
-      sincospif((float)t, &sn, &cs);
-      (which ? tG : tE)[b][ax][k] = cx<float>{cs, sn};
-    }
-  };
-  make_tables(m0, 0);
-  __syncthreads();
-  cx<float> v[N];
-  for (int t = 0; t < mc; ++t) {
-    const int mm = m0 + t, b = t & 1;
-    // A: E * fhat along z at (kx, ky) = (r0, r1), inverse along z
-    {
-      const cx<float> row = tE[b][0][r0] * tE[b][1][r1];
-#pragma unroll
-      for (int k = 0; k < N; ++k) v[k] = (row * tE[b][2][k]) * fhv[k];
-      fft_reg<N, +1>(v);
-#pragma unroll
-      for (int z = 0; z < N; ++z) W[r0][r1][z] = v[z];
-    }
-    wave_sync2();  // plane kx = r0 belongs to the 16 threads (r0, *) of one wave
-    // B: inverse along y at (kx, z) = (r0, r1)
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const int k = bitrev<N>(i);
-      v[k] = W[r0][k][r1];
-    }
-    fft_reg<N, +1>(v);
-#pragma unroll
-    for (int y = 0; y < N; ++y) W[r0][y][r1] = v[y];
-    __syncthreads();
-    // C: along x at (y, z) = (r0, r1): inverse, * f, forward
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const int k = bitrev<N>(i);
-      v[k] = W[k][r0][r1];
-    }
-    fft_reg<N, +1>(v);
-#pragma unroll
-    for (int x = 0; x < N; ++x) v[x] = fv[x] * v[x];
-    fft_reg<N, -1>(v);
-#pragma unroll
-    for (int k = 0; k < N; ++k) W[k][r0][r1] = v[k];
-    // the next pair's phase tables, into the other buffer: its last readers (phases A
-    // and E of the previous pair) are behind the barrier above
-    if (t + 1 < mc) make_tables(mm + 1, b ^ 1);
-    __syncthreads();
-    // D: forward along y at (kx, z) = (r0, r1)
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const int k = bitrev<N>(i);
-      v[k] = W[r0][k][r1];
-    }
-    fft_reg<N, -1>(v);
-#pragma unroll
-    for (int k = 0; k < N; ++k) W[r0][k][r1] = v[k];
-    wave_sync2();  // again within the plane's wave
-    // E: forward along z at (kx, ky) = (r0, r1), G * (.) and the sum over angles
-    // (the thread's own row: the next pair's phase A may overwrite it right after)
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const int k = bitrev<N>(i);
-      v[k] = W[r0][r1][k];
-    }
-    fft_reg<N, -1>(v);
-    const cx<float> grow = (float)A.wsig[mm] * (tG[b][0][r0] * tG[b][1][r1]);
-#pragma unroll
-    for (int k = 0; k < N; ++k) acc[k] = acc[k] + (grow * tG[b][2][k]) * v[k];
-  }
-  // radial weight and sp(a_j r_j |k|), then this (cell, j)'s partial spectrum
-  const float cj = (float)A.crad[ps * A.nr + j];
-  const int kx = kfreq(r0, N), ky = kfreq(r1, N);
-  cx<float>* out = A.part + ((long long)cell * nrs + jj) * NPTS + (long long)(r0 * N + r1) * N;
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    const int kz = kfreq(k, N);
-    out[k] = (cj * sp[kx * kx + ky * ky + kz * kz]) * acc[k];
-  }
+  fusedc3_body(A, cell, jj, threadIdx.x, W, tE, tG, sp);
 }
 
 // Scratch of the fused C2C forms per cell: FP64 f-hat, accumulator and two work arrays, then
 // the per-workgroup partial sums (one complex array per radial node, or per pair group), the
 // working-precision f-hat and f.
 template <typename T>
-static long long c2c_sizes(const Plan* P, long long batch, long long dflt, long long npts, long long parts,
-                           ScratchSizes* z) {
+static size_t c2c_cell_bytes(long long npts, long long parts) {
   const size_t b64 = (size_t)npts * sizeof(cx<double>);
-  const size_t aux_cell = (size_t)(parts + 1) * npts * sizeof(cx<T>) + (size_t)npts * sizeof(T);
-  const long long sub = fused_sub(P, batch, dflt, 4 * b64 + aux_cell);
+  return 4 * b64 + (size_t)(parts + 1) * npts * sizeof(cx<T>) + (size_t)npts * sizeof(T);
+}
+
+template <typename T>
+static void c2c_layout(long long sub, long long npts, long long parts, ScratchSizes* z) {
+  const size_t b64 = (size_t)npts * sizeof(cx<double>);
   z->fhat = sub * b64;
   z->acc = sub * b64;
   z->work = 2 * sub * b64;
-  z->aux = sub * aux_cell;
+  z->aux = sub * (c2c_cell_bytes<T>(npts, parts) - 4 * b64);
+}
+
+template <typename T>
+static long long c2c_sizes(const Plan* P, long long batch, long long dflt, long long npts, long long parts,
+                           ScratchSizes* z) {
+  const long long sub = fused_sub(P, batch, dflt, c2c_cell_bytes<T>(npts, parts));
+  c2c_layout<T>(sub, npts, parts, z);
   return sub;
 }
 
@@ -440,6 +170,107 @@ static int f2_wpc(const Plan* P) {
   return std::max(1, (P->nr * P->m + per - 1) / per);
 }
 
+// aux pieces of a C2C sub-batch of `sub` cells: the partial spectra (parts per cell), then
+// fhat / n and f in the kernel's type
+template <typename T>
+struct C2CPtrs {
+  cx<T>* part;
+  cx<T>* fh;
+  T* f;
+};
+
+template <typename T>
+static C2CPtrs<T> c2c_ptrs(const Plan* P, long long sub, long long npts, long long parts) {
+  C2CPtrs<T> r;
+  r.part = (cx<T>*)P->aux.ptr;
+  r.fh = (cx<T>*)((char*)P->aux.ptr + (size_t)sub * parts * npts * sizeof(cx<T>));
+  r.f = (T*)((char*)r.fh + (size_t)sub * npts * sizeof(cx<T>));
+  return r;
+}
+
+// prologue fhat = fft(f) in FP64 and the packing into the kernel's type
+template <typename T>
+static int c2c_front(Plan* P, const double* fc, long long nb, long long npts, const C2CPtrs<T>& b,
+                     hipStream_t st) {
+  int rc;
+  if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
+  const long long tot = nb * npts;
+  hipLaunchKernelGGL(k_f2_pack<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                     (const cx<double>*)P->fhat.ptr, fc, b.fh, b.f, tot, 1.0 / (double)npts);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+// after the pair loop: FP64 sum of each cell's partials in partial order, the FP64 epilogue
+template <typename T>
+static int c2c_back(Plan* P, const double* fc, double* qc, long long nb, long long npts, long long parts,
+                    const C2CPtrs<T>& b, const int* pc, int with_loss, double hb, hipStream_t st) {
+  const long long tot = nb * npts;
+  hipLaunchKernelGGL(k_f2_reduce<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, b.part,
+                     (cx<double>*)P->acc.ptr, nb, (int)parts, npts);
+  IB_HIP(hipGetLastError());
+  return staged_epilogue_fp64(P, fc, qc, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr, with_loss, hb, pc);
+}
+
+// plan fields every C2C argument struct carries
+template <typename Args>
+static void c2c_common(const Plan* P, Args& A, long long nb, const int* pc) {
+  A.pidx = pc;
+  A.batch = nb;
+  A.sigma = P->d_sigma;
+  A.rj = P->d_rj;
+  A.wsig = P->d_wsig;
+  A.crad = P->d_crad;
+  A.a = P->d_a;
+  A.nr = P->nr;
+  A.m = P->m;
+  A.nq = P->nq;
+  A.L = P->L;
+}
+
+static C3Args c3_args(const Plan* P, const C2CPtrs<float>& b, long long nb, const int* pc) {
+  C3Args A{};
+  c2c_common(P, A, nb, pc);
+  A.fh = b.fh;
+  A.f32 = b.f;
+  A.part = b.part;
+  A.spg = (const float*)P->d_spg;
+  A.ns = P->splits();
+  A.xg = nb >= 8;
+  return A;
+}
+
+template <typename T>
+static CWArgs<T> cw_args(const Plan* P, const C2CPtrs<T>& b, long long nb, const int* pc) {
+  CWArgs<T> A{};
+  c2c_common(P, A, nb, pc);
+  A.fh = b.fh;
+  A.f = b.f;
+  A.ephase = (const cx<T>*)P->d_ephase;
+  A.gphase = (const cx<T>*)P->d_gphase;
+  A.part = b.part;
+  A.spg = (const T*)P->d_spg;
+  A.ns = P->splits();
+  // the one-range form keeps the XCD-grouped numbering (without a split, each of a small
+  // batch's workgroups has a CU to itself either way)
+  A.xg = A.ns > 1 ? nb >= 8 : 1;
+  return A;
+}
+
+template <int N>
+static F2Args f2_args(const Plan* P, const C2CPtrs<float>& b, long long nb, const int* pc) {
+  F2Args A{};
+  c2c_common(P, A, nb, pc);
+  A.fh = b.fh;
+  A.ephase = (const cx<float>*)P->d_ephase;
+  A.gphase = (const cx<float>*)P->d_gphase;
+  A.f32 = b.f;
+  A.part = b.part;
+  A.wpc = f2_wpc<N>(P);
+  A.spg = (const float*)P->d_spg;
+  return A;
+}
+
 static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                        int with_loss, double hb, hipStream_t st) {
   using namespace c3;
@@ -447,348 +278,62 @@ static int run_fusedc3(Plan* P, const double* f, double* q, long long batch, con
   const long long sub = fc3_sizes(P, batch, &z);
   int rc;
   if ((rc = P->reserve(z))) return rc;
-  const int ns = P->splits();
-  const size_t part_b = (size_t)sub * P->nr * ns * NPTS * sizeof(cx<float>);
-  const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<float>);
-  cx<float>* part = (cx<float>*)P->aux.ptr;
-  cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + part_b);
-  float* f32 = (float*)((char*)fh32 + fh_b);
+  const long long parts = (long long)P->nr * P->splits();
+  const C2CPtrs<float> b = c2c_ptrs<float>(P, sub, NPTS, parts);
   for (long long c0 = 0; c0 < batch; c0 += sub) {
     const long long nb = std::min(sub, batch - c0);
     const double* fc = f + c0 * NPTS;
     const int* pc = pidx ? pidx + c0 : nullptr;
-    if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
-    const long long tot = nb * NPTS;
-    hipLaunchKernelGGL(k_f2_pack<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                       (const cx<double>*)P->fhat.ptr, fc, fh32, f32, tot, 1.0 / (double)NPTS);
-    IB_HIP(hipGetLastError());
-    C3Args A{};
-    A.fh = fh32;
-    A.f32 = f32;
-    A.part = part;
-    A.pidx = pc;
-    A.batch = nb;
-    A.sigma = P->d_sigma;
-    A.rj = P->d_rj;
-    A.wsig = P->d_wsig;
-    A.crad = P->d_crad;
-    A.a = P->d_a;
-    A.spg = (const float*)P->d_spg;
-    A.nr = P->nr;
-    A.m = P->m;
-    A.nq = P->nq;
-    A.ns = ns;
-    A.xg = nb >= 8;
-    A.L = P->L;
+    if ((rc = c2c_front<float>(P, fc, nb, NPTS, b, st))) return rc;
+    const C3Args A = c3_args(P, b, nb, pc);
     if ((rc = P->timed_begin(st))) return rc;
-    hipLaunchKernelGGL(k_fusedc3, dim3((unsigned)cell_grid(nb, P->nr * ns, A.xg)), dim3(THREADS), 0, st, A);
+    hipLaunchKernelGGL(k_fusedc3, dim3((unsigned)cell_grid(nb, (int)parts, A.xg)), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
-    hipLaunchKernelGGL(k_f2_reduce<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
-                       (cx<double>*)P->acc.ptr, nb, P->nr * ns, NPTS);
-    IB_HIP(hipGetLastError());
-    if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
-                                   with_loss, hb, pc)))
-      return rc;
+    if ((rc = c2c_back<float>(P, fc, q + c0 * NPTS, nb, NPTS, parts, b, pc, with_loss, hb, st))) return rc;
   }
   return IB_OK;
 }
 
-// One half of an N-point transform (DIR = +1 inverse, -1 forward) of a pencil
-// a[0..N-1], computed by lane h of the pencil's two lanes: on entry r[n] =
-// a[n] + (1 - 2h) a[n + N/2] (the first radix-2 decimation-in-frequency stage);
-// r[n] is multiplied by tw[n] = 1 (h = 0) or exp(2 pi i n / N) (h = 1, conjugated
-// for DIR = -1), read from LDS so that the twiddle constants take no registers,
-// and an N/2-point FFT leaves X[2k + h] in r[k].
-template <int DIR, int H, typename T>
-__device__ __forceinline__ void half_dif(cx<T> (&r)[H], const cx<T>* tw) {
-#pragma unroll
-  for (int n = 1; n < H; ++n) {
-    const cx<T> w = tw[n];
-    r[n] = DIR > 0 ? r[n] * w : r[n] * conj(w);
-  }
-  fft_reg<H, DIR>(r);
-}
-
-template <typename T>
-struct CWArgs {
-  const cx<T>* fh;      // [cell][N][N], fhat / n
-  const T* f;           // [cell][N][N]
-  const cx<T>* ephase;  // [pair][2][N] or null
-  const cx<T>* gphase;  // [param][pair][2][N] or null
-  cx<T>* part;          // [cell][nr][ns][N][N]
-  const int* pidx;
-  long long batch;
-  const double* sigma;
-  const double* rj;
-  const double* wsig;
-  const double* crad;
-  const double* a;
-  const T* spg;
-  int nr, m, nq;
-  int ns;  // workgroups per (cell, j), each over a contiguous range of the angles
-  int xg;  // workgroup numbering (cell_of_block)
-  double L;
-};
-
-// 2D grids whose pencil would not fit in one lane's registers next to the sum over
-// angles (FP32 64^2, FP64 32^2): one 2N-thread workgroup per (cell, radial node j)
-// loops over the M angles with the N x N complex work array in LDS; each pass
-// gives a pencil to two adjacent lanes (half_dif), so the sum over angles of a
-// row's N frequencies takes N/2 complex registers per lane.  Per pair: A rows
-// (E * fhat from L2, inverse along axis 1); B columns (inverse along axis 0, * f,
-// forward along axis 0; the column's two lanes are in one wave); C rows (forward
-// along axis 1, G * (.) and the sum).  Two workgroup barriers per pair.
-//
-// The work array is unpadded, element (r, c) at r N + (c ^ g(r)) with
-// g(r) = 2 floor(r / 2) ^ (3N/8) (r mod 2).  The lanes (p, h) of a wave access
-// (p, 2k + h) in phase A, (2k + h, p) at the end of phase B and (n, p), (p, n)
-// in the reads; with this permutation of each row every group of lanes served in
-// one LDS cycle touches distinct banks in all four patterns, whether the LDS
-// serves 128 or 256 bytes per cycle (a padded row of N + 1 elements puts lanes
-// (p, 1) and (p + 1, 0) on one bank in both write patterns).
-template <int N>
-__device__ __forceinline__ constexpr int wsw(int r) {
-  return (2 * (r >> 1)) ^ ((3 * N / 8) * (r & 1));
-}
-
-// SPLIT = false is the one-range form (ns = 1): its FP64 instance keeps two waves per SIMD,
-// which the split form's extra scalar values would cost it.
 template <typename T, int N, bool SPLIT>
 __global__ __launch_bounds__(2 * N) void k_fusedw(CWArgs<T> A) {
-  constexpr int H = N / 2, THREADS = 2 * N, SWH = 3 * N / 8;
-  constexpr long long NPTS = (long long)N * N;
   __shared__ cx<T> W[N * N];
   __shared__ cx<T> tE[2][2][N];
   __shared__ cx<T> tG[2][2][N];
-  // ones, then exp(2 pi i n / N); the second row starts one element later than H, so that
-  // the two lanes of a pencil (reading entry n of row 0 and of row 1 together) use different banks
-  __shared__ cx<T> twd[2][H + 1];
-  const int tid = threadIdx.x;
-  const long long id = blockIdx.x;
-  const int ns = SPLIT ? A.ns : 1;
-  const int nrs = A.nr * ns;
+  __shared__ cx<T> twd[2][N / 2 + 1];
   long long cell;
-  int jj;  // partial index: radial node j, angle range asp
-  cell_of_block(id, nrs, SPLIT ? A.xg : 1, cell, jj);
-  const int j = jj / ns, asp = jj % ns;
+  int jj;
+  cell_of_block(blockIdx.x, A.nr * (SPLIT ? A.ns : 1), SPLIT ? A.xg : 1, cell, jj);
   if (cell >= A.batch) return;  // uniform over the workgroup
-  const int m0 = SPLIT ? (asp * A.m) / ns : 0;  // angles m0 .. mend - 1
-  const int mend = SPLIT ? ((asp + 1) * A.m) / ns : A.m;
-  const int ps = A.pidx ? A.pidx[cell] : 0;
-  if (tid < N) {
-    const int n = tid % H;
-    twd[tid / H][n] = tid < H ? cx<T>{(T)1, (T)0}
-                              : cx<T>{(T)TW64[n * (64 / N)][0], (T)TW64[n * (64 / N)][1]};
-  }
-  const int p = tid >> 1, h = tid & 1;  // pencil and half
-  const cx<T>* tw = twd[h];
-  // swizzled offsets: row p's element 2k + h is at p N + (2k ^ hg); row 2k + h's element p is
-  // at h N + 2k N + (2k ^ ph).  Each phase takes its offsets from an opaque copy, so the
-  // compiler forms the loop-invariant addresses next to their accesses instead of holding
-  // all of them in registers across the pair loop.
-  const int hg0 = h ^ wsw<N>(p);
-  const int ph0 = p ^ (SWH * h);
-  auto opaque = [](int v) {
-    asm volatile("" : "+v"(v));
-    return v;
-  };
-  const T sg = h ? (T)-1 : (T)1;
-  const double aj = A.a[ps * A.nr + j] * A.rj[j];
-  const double rpl = M_PI * A.rj[j] / A.L;
-  const int npairs = A.nr * A.m;
-  const cx<T>* fh = A.fh + cell * NPTS;
-  const T* fc = A.f + cell * NPTS;
-  // phase factors of pair (j, mm) along each axis: from the plan's tables when it
-  // has them, else FP64 sincos (inelastic_collisions.py:97-105)
-  auto load_tables = [&](int mm, int b) {
-    const int pair = j * A.m + mm;
-    for (int e = tid; e < 4 * N; e += THREADS) {
-      const int which = e / (2 * N), ax = (e / N) % 2, k = e % N;
-      cx<T> val;
-      if (which == 0 && A.ephase) {
-        val = A.ephase[((long long)pair * 2 + ax) * N + k];
-      } else if (which == 1 && A.gphase) {
-        val = A.gphase[(((long long)ps * npairs + pair) * 2 + ax) * N + k];
-      } else {
-        const double s = A.sigma[mm * 2 + ax];
-        const double kf = (double)kfreq(k, N);
-        const double th = which == 0 ? -(rpl * s * kf) : (aj * s * kf);
-        double sn, cs;
-        sincos(th, &sn, &cs);
-        val = cx<T>{(T)cs, (T)sn};
-      }
-      (which ? tG : tE)[b][ax][k] = val;
-    }
-  };
-  auto split_row = [&](cx<T> (&r)[H], const cx<T>& a0, const cx<T>& a1, int n) {
-    r[n] = cx<T>{a0.re + sg * a1.re, a0.im + sg * a1.im};
-  };
-  // FP32 with the plan's tables: the next pair's entries of this thread (one E and
-  // one G value: the loop of load_tables visits e = tid and e = tid + 2N) are loaded
-  // at the start of phase B and stored to LDS at its end, so their L2 latency
-  // overlaps B (3.5% at 64^2; no gain in FP64)
-  const bool pre = sizeof(T) == 4 && A.ephase && A.gphase;
-  const int tax = (tid / N) % 2, tk = tid % N;
-  cx<T> nt[2];
-  auto fetch = [&](int mm) {
-    const int pair = j * A.m + mm;
-    nt[0] = A.ephase[((long long)pair * 2 + tax) * N + tk];
-    nt[1] = A.gphase[(((long long)ps * npairs + pair) * 2 + tax) * N + tk];
-  };
-  cx<T> acc[H];
-#pragma unroll
-  for (int k = 0; k < H; ++k) acc[k] = cx<T>{(T)0, (T)0};
-  load_tables(m0, 0);
-  __syncthreads();
-  cx<T> r[H];
-  for (int mm = m0; mm < mend; ++mm) {
-    const int b = (mm - m0) & 1;
-    // A: row p = kx: E * fhat, inverse along axis 1 (ky -> y); the row factor
-    // E_x(kx) is constant along the row and applied to the outputs
-    {
-      const cx<T>* fr = fh + (long long)p * N;
-#pragma unroll
-      for (int n = 0; n < H; ++n) {
-        split_row(r, tE[b][1][n] * fr[n], tE[b][1][n + H] * fr[n + H], n);
-        if ((n & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // 16 loads in flight at a time
-      }
-      half_dif<+1>(r, tw);
-      const cx<T> ex = tE[b][0][p];
-      const int hg = opaque(hg0), rowp = opaque(p * N);
-#pragma unroll
-      for (int k = 0; k < H; ++k) W[rowp + ((2 * k) ^ hg)] = ex * r[k];
-    }
-    __syncthreads();
-    // B: column p = y: inverse along axis 0 (kx -> x), * f, forward along axis 0;
-    // the next pair's phase tables (their buffer's last readers, phases A and C of
-    // the previous pair, are behind the barrier above)
-    {
-      if (pre && mm + 1 < mend) fetch(mm + 1);
-      int pc = opaque(p);
-#pragma unroll
-      for (int n = 0; n < H; ++n)
-        split_row(r, W[n * N + (pc ^ wsw<N>(n))], W[(n + H) * N + (pc ^ wsw<N>(n + H))], n);
-      half_dif<+1>(r, tw);
-      // 32-bit offset made opaque per pair, so the addresses are formed next to
-      // their loads instead of being held in registers across the pair loop
-      int toff = h * N + p;
-      asm volatile("" : "+v"(toff));
-#pragma unroll
-      for (int k = 0; k < H; ++k) {
-        r[k] = fc[2 * k * N + toff] * r[k];
-        if ((k & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-      }
-      wave_sync2();  // both lanes of the column have read it
-      int ph = opaque(ph0), rowh = opaque(h * N);
-#pragma unroll
-      for (int k = 0; k < H; ++k) W[rowh + 2 * k * N + ((2 * k) ^ ph)] = r[k];
-      wave_sync2();  // ... and written their halves
-      pc = opaque(p);
-#pragma unroll
-      for (int n = 0; n < H; ++n)
-        split_row(r, W[n * N + (pc ^ wsw<N>(n))], W[(n + H) * N + (pc ^ wsw<N>(n + H))], n);
-      half_dif<-1>(r, tw);
-      wave_sync2();
-      ph = opaque(ph0);
-      rowh = opaque(h * N);
-#pragma unroll
-      for (int k = 0; k < H; ++k) W[rowh + 2 * k * N + ((2 * k) ^ ph)] = r[k];
-    }
-    if (mm + 1 < mend) {
-      if (pre) {
-        tE[b ^ 1][tax][tk] = nt[0];
-        tG[b ^ 1][tax][tk] = nt[1];
-      } else {
-        load_tables(mm + 1, b ^ 1);
-      }
-    }
-    __syncthreads();
-    // C: row p = kx: forward along axis 1 (y -> ky), G * (.), sum over angles (the
-    // row's own two lanes are the only readers, so the next pair's phase A may
-    // overwrite it without a barrier)
-    {
-      const int gp = opaque(wsw<N>(p)), rowp = opaque(p * N);
-#pragma unroll
-      for (int n = 0; n < H; ++n) split_row(r, W[rowp + (n ^ gp)], W[rowp + ((n + H) ^ gp)], n);
-      half_dif<-1>(r, tw);
-      const cx<T> gx = (T)A.wsig[mm] * tG[b][0][p];
-#pragma unroll
-      for (int k = 0; k < H; ++k) acc[k] = acc[k] + (gx * tG[b][1][2 * k + h]) * r[k];
-      wave_sync2();
-    }
-  }
-  // radial weight and sp(a_j r_j |k|), then this (cell, j, angle range)'s partial spectrum
-  const T cj = (T)A.crad[ps * A.nr + j];
-  const T* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
-  const int kx = kfreq(p, N);
-  cx<T>* out = A.part + ((long long)cell * nrs + jj) * NPTS + (long long)p * N;
-#pragma unroll
-  for (int k = 0; k < H; ++k) {
-    const int ky = kfreq(2 * k + h, N);
-    out[2 * k + h] = (cj * sp[kx * kx + ky * ky]) * acc[k];
-  }
+  fusedw_body<T, N, SPLIT>(A, cell, jj, threadIdx.x, W, tE, tG, twd);
 }
 
 template <typename T, int N>
 static int run_fusedw(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                       int with_loss, double hb, hipStream_t st) {
   constexpr long long NPTS = (long long)N * N;
-  ScratchSizes z;
   const int ns = P->splits();
-  const long long sub = c2c_sizes<T>(P, batch, 1024, NPTS, (long long)P->nr * ns, &z);
+  const long long parts = (long long)P->nr * ns;
+  ScratchSizes z;
+  const long long sub = c2c_sizes<T>(P, batch, 1024, NPTS, parts, &z);
   int rc;
   if ((rc = P->reserve(z))) return rc;
-  const size_t part_b = (size_t)sub * P->nr * ns * NPTS * sizeof(cx<T>);
-  const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<T>);
-  cx<T>* part = (cx<T>*)P->aux.ptr;
-  cx<T>* fhn = (cx<T>*)((char*)P->aux.ptr + part_b);
-  T* fT = (T*)((char*)fhn + fh_b);
+  const C2CPtrs<T> b = c2c_ptrs<T>(P, sub, NPTS, parts);
   for (long long c0 = 0; c0 < batch; c0 += sub) {
     const long long nb = std::min(sub, batch - c0);
     const double* fc = f + c0 * NPTS;
     const int* pc = pidx ? pidx + c0 : nullptr;
-    if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
-    const long long tot = nb * NPTS;
-    hipLaunchKernelGGL(k_f2_pack<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                       (const cx<double>*)P->fhat.ptr, fc, fhn, fT, tot, 1.0 / (double)NPTS);
-    IB_HIP(hipGetLastError());
-    CWArgs<T> A{};
-    A.fh = fhn;
-    A.f = fT;
-    A.ephase = (const cx<T>*)P->d_ephase;
-    A.gphase = (const cx<T>*)P->d_gphase;
-    A.part = part;
-    A.pidx = pc;
-    A.batch = nb;
-    A.sigma = P->d_sigma;
-    A.rj = P->d_rj;
-    A.wsig = P->d_wsig;
-    A.crad = P->d_crad;
-    A.a = P->d_a;
-    A.spg = (const T*)P->d_spg;
-    A.nr = P->nr;
-    A.m = P->m;
-    A.nq = P->nq;
-    A.ns = ns;
-    A.L = P->L;
-    // the one-range form keeps the XCD-grouped numbering (without a split, each of a small
-    // batch's workgroups has a CU to itself either way)
-    A.xg = ns > 1 ? nb >= 8 : 1;
+    if ((rc = c2c_front<T>(P, fc, nb, NPTS, b, st))) return rc;
+    const CWArgs<T> A = cw_args<T>(P, b, nb, pc);
     if ((rc = P->timed_begin(st))) return rc;
-    const unsigned grid = (unsigned)cell_grid(nb, P->nr * ns, A.xg);
+    const unsigned grid = (unsigned)cell_grid(nb, (int)parts, A.xg);
     if (ns > 1)
       hipLaunchKernelGGL((k_fusedw<T, N, true>), dim3(grid), dim3(2 * N), 0, st, A);
     else
       hipLaunchKernelGGL((k_fusedw<T, N, false>), dim3(grid), dim3(2 * N), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
-    hipLaunchKernelGGL(k_f2_reduce<T>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
-                       (cx<double>*)P->acc.ptr, nb, P->nr * ns, NPTS);
-    IB_HIP(hipGetLastError());
-    if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
-                                   with_loss, hb, pc)))
-      return rc;
+    if ((rc = c2c_back<T>(P, fc, q + c0 * NPTS, nb, NPTS, parts, b, pc, with_loss, hb, st))) return rc;
   }
   return IB_OK;
 }
@@ -796,62 +341,99 @@ static int run_fusedw(Plan* P, const double* f, double* q, long long batch, cons
 template <int N>
 static int run_fused2(Plan* P, const double* f, double* q, long long batch, const int* pidx,
                       int with_loss, double hb, hipStream_t st) {
-  constexpr int G = F2Shape<N>::G, THREADS = F2Shape<N>::THREADS;
+  constexpr int THREADS = F2Shape<N>::THREADS;
   constexpr long long NPTS = (long long)N * N;
-  const int npairs = P->nr * P->m;
-  // workgroups per cell fixed by the plan (16 rounds of G pairs each), not by the
-  // batch size: a cell's FP32 summation order, and so its result, is the same in
-  // any batch
+  // workgroups per cell fixed by the plan (its rounds of G pairs each), not by the batch
+  // size: a cell's FP32 summation order, and so its result, is the same in any batch
   const int wpc = f2_wpc<N>(P);
   ScratchSizes z;
   const long long sub = c2c_sizes<float>(P, batch, 4096, NPTS, wpc, &z);
   int rc;
   if ((rc = P->reserve(z))) return rc;
-  const size_t part_b = (size_t)sub * wpc * NPTS * sizeof(cx<float>);
-  const size_t fh_b = (size_t)sub * NPTS * sizeof(cx<float>);
-  cx<float>* part = (cx<float>*)P->aux.ptr;
-  cx<float>* fh32 = (cx<float>*)((char*)P->aux.ptr + part_b);
-  float* f32 = (float*)((char*)fh32 + fh_b);
+  const C2CPtrs<float> b = c2c_ptrs<float>(P, sub, NPTS, wpc);
   for (long long c0 = 0; c0 < batch; c0 += sub) {
     const long long nb = std::min(sub, batch - c0);
     const double* fc = f + c0 * NPTS;
     const int* pc = pidx ? pidx + c0 : nullptr;
-    if ((rc = staged_prologue_fp64(P, fc, nb, P->fhat.ptr, P->work.ptr))) return rc;
-    const long long tot = nb * NPTS;
-    hipLaunchKernelGGL(k_f2_pack<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
-                       (const cx<double>*)P->fhat.ptr, fc, fh32, f32, tot, 1.0 / (double)NPTS);
-    IB_HIP(hipGetLastError());
-    F2Args A{};
-    A.fh = fh32;
-    A.ephase = (const cx<float>*)P->d_ephase;
-    A.gphase = (const cx<float>*)P->d_gphase;
-    A.f32 = f32;
-    A.part = part;
-    A.pidx = pc;
-    A.batch = nb;
-    A.wpc = wpc;
-    A.sigma = P->d_sigma;
-    A.rj = P->d_rj;
-    A.wsig = P->d_wsig;
-    A.crad = P->d_crad;
-    A.a = P->d_a;
-    A.spg = (const float*)P->d_spg;
-    A.nr = P->nr;
-    A.m = P->m;
-    A.nq = P->nq;
-    A.L = P->L;
+    if ((rc = c2c_front<float>(P, fc, nb, NPTS, b, st))) return rc;
+    const F2Args A = f2_args<N>(P, b, nb, pc);
     if ((rc = P->timed_begin(st))) return rc;
     hipLaunchKernelGGL(k_fused2<N>, dim3((unsigned)(nb * wpc)), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
-    hipLaunchKernelGGL(k_f2_reduce<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, part,
-                       (cx<double>*)P->acc.ptr, nb, wpc, NPTS);
-    IB_HIP(hipGetLastError());
-    if ((rc = staged_epilogue_fp64(P, fc, q + c0 * NPTS, nb, P->acc.ptr, P->fhat.ptr, P->work.ptr,
-                                   with_loss, hb, pc)))
-      return rc;
+    if ((rc = c2c_back<float>(P, fc, q + c0 * NPTS, nb, NPTS, wpc, b, pc, with_loss, hb, st))) return rc;
   }
   return IB_OK;
+}
+
+// Grouped launch (group.h), FP32 C2C forms: the shape's kind, points per cell and partial
+// spectra per cell, or 0 when the plan has no such form
+static int f2_group_kind(const Plan* P, long long* npts, long long* parts) {
+  if (P->precision == IB_PREC_FP64) return -1;
+  const int ns = P->splits();
+  if (P->dim == 3 && P->n == 16) {
+    *npts = c3::NPTS;
+    *parts = (long long)P->nr * ns;
+    return GK_C3;
+  }
+  if (P->dim != 2) return -1;
+  switch (P->n) {
+    case 16: *npts = 256; *parts = f2_wpc<16>(P); return GK_F2_16;
+    case 32: *npts = 1024; *parts = f2_wpc<32>(P); return GK_F2_32;
+    case 64: *npts = 4096; *parts = (long long)P->nr * ns; return ns > 1 ? GK_W64S : GK_W64;
+    default: return -1;
+  }
+}
+
+// The whole batch as one segment when its scratch fits the plan's budget (*ok = 0
+// otherwise: the plan then runs alone).
+int f2_group_front(Plan* P, const double* f, long long batch, const int* pidx, hipStream_t st,
+                   GroupSeg* seg, int* ok) {
+  *ok = 0;
+  long long npts = 0, parts = 0;
+  const int kind = f2_group_kind(P, &npts, &parts);
+  if (kind < 0 || (size_t)batch * c2c_cell_bytes<float>(npts, parts) > P->scratch_budget) return IB_OK;
+  ScratchSizes z;
+  c2c_layout<float>(batch, npts, parts, &z);
+  int rc;
+  if ((rc = P->reserve(z))) return rc;
+  const C2CPtrs<float> b = c2c_ptrs<float>(P, batch, npts, parts);
+  if ((rc = c2c_front<float>(P, f, batch, npts, b, st))) return rc;
+  seg->kind = kind;
+  const int ns = P->splits();
+  switch (kind) {
+    case GK_C3:
+      seg->c3 = c3_args(P, b, batch, pidx);
+      seg->units = batch * P->nr;
+      break;
+    case GK_W64:
+    case GK_W64S:
+      seg->w = cw_args<float>(P, b, batch, pidx);
+      seg->units = batch * P->nr;
+      break;
+    case GK_F2_16:
+      seg->f2 = f2_args<16>(P, b, batch, pidx);
+      seg->units = batch * seg->f2.wpc;
+      break;
+    default:
+      seg->f2 = f2_args<32>(P, b, batch, pidx);
+      seg->units = batch * seg->f2.wpc;
+      break;
+  }
+  seg->blocks = group_blocks(kind, seg->units, ns, 0);
+  *ok = 1;
+  return IB_OK;
+}
+
+int f2_group_back(Plan* P, const double* f, double* q, long long batch, const int* pidx, int with_loss,
+                  double hb, hipStream_t st) {
+  long long npts = 0, parts = 0;
+  if (f2_group_kind(P, &npts, &parts) < 0) {
+    set_error("grouped fused launch: plan has no grouped form");
+    return IB_ERR_ARG;
+  }
+  return c2c_back<float>(P, f, q, batch, npts, parts, c2c_ptrs<float>(P, batch, npts, parts), pidx, with_loss,
+                         hb, st);
 }
 
 int fused2_sizes(const Plan* P, long long batch, ScratchSizes* z) {

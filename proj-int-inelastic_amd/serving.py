@@ -115,7 +115,7 @@ _SERIAL = itertools.count()
 class _Class:
     """Problems sharing one collision plan and one integrator."""
 
-    def __init__(self, key, proto, autotune=False):
+    def __init__(self, key, proto, autotune=False, scratch_budget=1 << 30):
         t = dev.require_gpu()
         # graphs are keyed by (serial, version): a class built after another was released
         # never matches the released class's graph, whatever id() Python gives it
@@ -132,6 +132,9 @@ class _Class:
             cfg["spherical-design-rule"] = {"ssrule": "lebedev", "nsphr": proto.m}
         self.sm = SpectralMesh(cfg)
         self.coll = FSInelasticVHSCollision(cfg, self.sm, precision=proto.precision)
+        # a class's whole batch fits its plan's scratch, so its pair loops join the grouped
+        # launch of its group (admission counts the scratch against the HBM budget)
+        self.coll.set_tuning(scratch_budget=scratch_budget)
         self.vshape = (proto.n,) * d
         self.active = []
         self.f = t.empty((0,) + self.vshape, dtype=t.float64, device="cuda")
@@ -370,7 +373,8 @@ class Server:
     def _class(self, key, proto):
         cls = self.classes.get(key)
         if cls is None:
-            cls = self.classes[key] = _Class(key, proto, self.autotune)
+            cls = self.classes[key] = _Class(key, proto, self.autotune,
+                                             scratch_budget=max(1 << 30, self.budget // 4))
             self.reserved[key] = 0
         return cls
 
