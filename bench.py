@@ -619,13 +619,14 @@ def main():
     ap.add_argument("--check-cells", type=int, default=2)
     ap.add_argument("--cpu-cells", type=int, default=36)
     ap.add_argument("--mode", choices=["bench", "profile", "profile2d", "profile2d32fp64", "profile2d64",
-                                       "profile3d16", "uneven", "copy"],
+                                       "profile3d16", "profile3d32fp64", "uneven", "copy"],
                     default="bench",
                     help="profile: FP32 form only on --cells cells (for rocprofv3); "
                          "profile2d: the 2D 32^2 FP32 form only on --cells-2d cells; "
                          "profile2d32fp64: the 2D 32^2 FP64 form only on --cells-2d cells; "
                          "profile2d64: the 2D 64^2 FP32 form only on --cells-2d64 cells; "
                          "profile3d16: the 3D 16^3 FP32 form only on --cells-3d16 cells; "
+                         "profile3d32fp64: the plain FP64 form of the 3D 32^3 batch only on --cells cells; "
                          "uneven: the FP32 uneven batch only (--uneven-problems, default shapes, "
                          "no graph replay; --per-plan-fused for one fused launch per plan); "
                          "copy: a plain 1 GiB device copy (PMC byte-counter calibration)")
@@ -714,11 +715,12 @@ def main():
     f, e = make_batch(args.cells, sm.L, seed=0)
     f_dev = torch.from_numpy(f).cuda()
     c32 = FSInelasticVHSCollision(cfg, sm, precision="fp32")
-    if args.mode == "profile":
+    if args.mode in ("profile", "profile3d32fp64"):
+        cp = c32 if args.mode == "profile" else FSInelasticVHSCollision(cfg, sm, precision="fp64")
         for _ in range(args.warmup + args.steps):
-            c32.collide(f_dev, with_loss=True, e=e, project=True)
+            cp.collide(f_dev, with_loss=True, e=e, project=True)
         torch.cuda.synchronize()
-        print(json.dumps({"mode": "profile", "cells": args.cells, "launch_steps": args.warmup + args.steps}))
+        print(json.dumps({"mode": args.mode, "cells": args.cells, "launch_steps": args.warmup + args.steps}))
         return
     t32, w32, q32 = time_path(c32, f_dev, e, args.steps, args.warmup, timed_kernel=True)
     kms, klaunch, kcells = c32.get_timing()

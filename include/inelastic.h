@@ -153,7 +153,8 @@ int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, 
 
 /* Mode of ib_collide_group for FP32 plans with a fused form (3D 32^3 and 16^3, 2D 16^2, 32^2
  * and 64^2): 1 (default) runs the quadrature-pair loops of all of them as ONE grouped kernel
- * launch per call (plans whose whole batch fits their scratch budget; the others run alone),
+ * launch per call (plans whose whole batch fits their scratch budget, when there are at least
+ * two such plans; the others, or a single such plan, run their own kernels),
  * with each plan's FP64 prologue, Nyquist-plane term, FP64 reduce and epilogue around it; 0
  * runs one fused launch per plan.  Both modes give bit-identical results.  Process-wide; set it
  * between calls, not while another thread is inside ib_collide_group.  New: the grouped launch

@@ -104,6 +104,7 @@ int collide_group(Plan** P, int np, const double* const* f, double* const* q,
                   const long long* batch, const int* const* pidx, int with_loss, double hb,
                   hipStream_t st);
 // grouped launch of the fused pair loops (group.h, collide_fused.hip, collide_group.hip)
+int fused_group_fits(const Plan* P, long long batch);
 int fused_group_front(Plan* P, const double* f, long long batch, const int* pidx, hipStream_t st,
                       GroupSeg* seg, int* ok, int* which);
 int fused_group_back(Plan* P, int which, const double* f, double* q, long long batch, const int* pidx,
@@ -505,7 +506,13 @@ int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, 
       }
   }
   // FP32 plans with a fused form whose whole batch fits their scratch budget: their pair
-  // loops run as segments of one grouped launch (group.h) between their front and back work
+  // loops run as segments of one grouped launch (group.h) between their front and back work,
+  // when there are at least two of them (one alone runs its own kernel: the same results,
+  // without the combined kernel's costs)
+  int n_fit = 0;
+  if (g_group_fused && prec != IB_PREC_FP64)
+    for (int i = 0; i < nplans; ++i) n_fit += batch[i] > 0 && fused_group_fits(plans[i], batch[i]);
+  const bool group_fused = n_fit >= 2;
   std::vector<GroupSeg> segs;
   struct Grouped {
     Plan* p;
@@ -517,7 +524,7 @@ int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, 
     if (batch[i] == 0) continue;
     p->stream = st;
     const int* pi = param_idx ? param_idx[i] : nullptr;
-    if (g_group_fused && prec != IB_PREC_FP64) {
+    if (group_fused) {
       GroupSeg seg{};
       int ok = 0, which = 0;
       int rc = fused_group_front(p, f[i], batch[i], pi, st, &seg, &ok, &which);

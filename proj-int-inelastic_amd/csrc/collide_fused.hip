@@ -305,6 +305,20 @@ int f2_group_front(Plan* P, const double* f, long long batch, const int* pidx, h
 int f2_group_back(Plan* P, const double* f, double* q, long long batch, const int* pidx, int with_loss,
                   double hb, hipStream_t st);
 
+int f2_group_fits(const Plan* P, long long batch);
+
+// Whether a plan's whole batch can be one segment of a grouped launch (an FP32 fused form whose
+// scratch for the whole batch fits the plan's budget); no work is queued.
+int fused_group_fits(const Plan* P, long long batch) {
+  if (P->precision == IB_PREC_FP64 || batch <= 0) return 0;
+  const int k = fused_kind(P);
+  if (k == 1) {
+    size_t pb, fb;
+    return (size_t)batch * f3_cell_bytes(P, &pb, &fb) <= P->scratch_budget;
+  }
+  return k == 2 ? f2_group_fits(P, batch) : 0;
+}
+
 // A plan's part of a grouped launch: *ok = 1 when its whole batch became one segment (the
 // front work queued), with *which naming the engine for fused_group_back.
 int fused_group_front(Plan* P, const double* f, long long batch, const int* pidx, hipStream_t st,

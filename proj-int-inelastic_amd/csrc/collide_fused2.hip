@@ -385,6 +385,12 @@ static int f2_group_kind(const Plan* P, long long* npts, long long* parts) {
   }
 }
 
+int f2_group_fits(const Plan* P, long long batch) {
+  long long npts = 0, parts = 0;
+  return f2_group_kind(P, &npts, &parts) >= 0 &&
+         (size_t)batch * c2c_cell_bytes<float>(npts, parts) <= P->scratch_budget;
+}
+
 // The whole batch as one segment when its scratch fits the plan's budget (*ok = 0
 // otherwise: the plan then runs alone).
 int f2_group_front(Plan* P, const double* f, long long batch, const int* pidx, hipStream_t st,

@@ -132,9 +132,11 @@ class _Class:
             cfg["spherical-design-rule"] = {"ssrule": "lebedev", "nsphr": proto.m}
         self.sm = SpectralMesh(cfg)
         self.coll = FSInelasticVHSCollision(cfg, self.sm, precision=proto.precision)
-        # a class's whole batch fits its plan's scratch, so its pair loops join the grouped
-        # launch of its group (admission counts the scratch against the HBM budget)
-        self.coll.set_tuning(scratch_budget=scratch_budget)
+        # FP32: a class's whole batch fits its plan's scratch, so its pair loops join the
+        # grouped launch of its group (admission counts the scratch against the HBM budget);
+        # FP64 plans keep the default budget, by which the staged engine sizes its sub-batches
+        if proto.precision != "fp64":
+            self.coll.set_tuning(scratch_budget=scratch_budget)
         self.vshape = (proto.n,) * d
         self.active = []
         self.f = t.empty((0,) + self.vshape, dtype=t.float64, device="cuda")
