@@ -248,6 +248,9 @@ class _Group:
         self.use_graph = use_graph
         self._graph = None
         self._graph_key = None
+        # each group steps on its own stream (default priority), so that the groups of one
+        # tick run side by side on the device
+        self.stream = dev.torch().cuda.Stream()
 
     def df(self, h):
         cls = self.classes
@@ -342,6 +345,8 @@ class Server:
         self.reserved = {}   # class key -> plan scratch bytes counted in `used`
         self.ticks = 0
         self.peak_used = 0
+        self.tick_s = 2e-3   # shortest tick while requests are still to arrive (seconds)
+        self._step_s = 0.0   # running estimate of one outer step's duration
 
     def submit(self, req):
         """Queue a request.  Requests that no state of the server could admit raise
@@ -449,16 +454,31 @@ class Server:
                 nxt = min(r.arrival for r in self.queue)
                 time.sleep(max(0.0, min(nxt - self._now(), 0.01)))
                 continue
+            # Steps this tick: up to the first completion (no admission or retirement falls
+            # inside a tick); while requests are still to arrive, also about as long as the wait
+            # for the next arrival (at least one step), so arrivals are admitted as before.
+            steps = min(a.steps_left for c in busy for a in c.active)
+            if realtime and self.queue:
+                horizon = max(self.tick_s, min(r.arrival for r in self.queue) - self._now())
+                steps = min(steps, max(1, int(horizon / max(self._step_s, 1e-5))))
+            main = t.cuda.current_stream()
+            tick_start = time.perf_counter()
             for gk in sorted({c.proto.group_key() for c in busy}, key=repr):
                 grp = self.groups.setdefault(gk, _Group(gk, self.use_graph))
                 grp.classes = [c for c in busy if c.proto.group_key() == gk]
-                grp.step()
+                grp.stream.wait_stream(main)  # the classes' tensors were rebuilt on `main`
+                with t.cuda.stream(grp.stream):
+                    for _ in range(steps):
+                        grp.step()
             self.ticks += 1
             for cls in busy:
                 for a in cls.active:
-                    a.steps_left -= 1
-                    a.t += cls.proto.dt
+                    for _ in range(steps):
+                        a.steps_left -= 1
+                        a.t += cls.proto.dt
             t.cuda.synchronize()
+            dt_tick = (time.perf_counter() - tick_start) / steps
+            self._step_s = dt_tick if self._step_s <= 0 else 0.8 * self._step_s + 0.2 * dt_tick
             done_t = self._now()
             for cls in busy:
                 done = [a for a in cls.active if a.steps_left <= 0]
