@@ -630,6 +630,8 @@ def main():
                          "uneven: the FP32 uneven batch only (--uneven-problems, default shapes, "
                          "no graph replay; --per-plan-fused for one fused launch per plan); "
                          "copy: a plain 1 GiB device copy (PMC byte-counter calibration)")
+    ap.add_argument("--serve-autotune", action="store_true",
+                    help="serving: classes take their autotuned launch and tile shapes")
     ap.add_argument("--per-plan-fused", action="store_true",
                     help="uneven mode: one fused launch per plan instead of the grouped launch")
     ap.add_argument("--no-cpu", action="store_true")
@@ -871,7 +873,7 @@ def main():
                                       % (args.serve_rate, args.serve_requests)}
         sres = {}
         for prec in (("fp32", "fp64") if not args.no_fp64 else ("fp32",)):
-            srv = serving.Server(hbm_budget=32 << 30)
+            srv = serving.Server(hbm_budget=32 << 30, autotune=args.serve_autotune)
             sres[prec] = srv.run(serve_sweep_requests(prec, args.serve_requests, args.serve_rate,
                                                       seed=args.serve_seed, full_time=True),
                                  realtime=True)

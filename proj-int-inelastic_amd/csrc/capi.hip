@@ -558,6 +558,11 @@ int ib_collide_group(int nplans, ib_plan* const* plans, const double* const* f, 
                            gi.data() + s0, with_loss, heat_bath, st);
     if (rc) return rc;
   }
+  // the 3D 32^3 segments first: their workgroups run longest, and the small shapes' short
+  // workgroups then fill the launch's tail
+  std::stable_sort(segs.begin(), segs.end(), [](const GroupSeg& a, const GroupSeg& b) {
+    return (a.kind == GK_F3 || a.kind == GK_F3K) > (b.kind == GK_F3 || b.kind == GK_F3K);
+  });
   for (size_t s0 = 0; s0 < segs.size(); s0 += kMaxGroupSegs) {
     const int n = (int)std::min<size_t>(kMaxGroupSegs, segs.size() - s0);
     int rc = launch_fused_group(segs.data() + s0, n, st);
