@@ -96,6 +96,13 @@ def test_collision_entry_points_reject(pkg):
     assert lib.ib_plan_destroy(None) == 0
 
 
+def test_group_mode_rejects(pkg):
+    lib = pkg._lib.load()
+    assert lib.ib_set_group_mode(2) == -1
+    assert b"group mode" in lib.ib_last_error()
+    assert lib.ib_set_group_mode(-1) == -1
+
+
 @pytest.mark.parametrize("split,rounds", [(0, 0), (65, 0), (1, -1), (1, 1025)])
 def test_tile_shape_rejects(pkg, split, rounds):
     """Tile shapes outside [1, 64] splits and [0, 1024] rounds are refused before the plan
