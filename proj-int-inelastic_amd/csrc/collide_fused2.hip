@@ -29,6 +29,7 @@
 #include <algorithm>
 
 #include "common.h"
+#include "edge2_body.h"
 #include "group.h"
 #include "plan.h"
 
@@ -188,6 +189,39 @@ static C2CPtrs<T> c2c_ptrs(const Plan* P, long long sub, long long npts, long lo
   return r;
 }
 
+// 2D 16^2 / 32^2 forms: prologue and epilogue in one launch each (edge2_body.h)
+template <int N>
+static constexpr bool edges_fused() {
+  return N <= 32;
+}
+
+template <typename T, int N>
+static int front2d(Plan* P, const double* fc, long long nb, const C2CPtrs<T>& b, hipStream_t st) {
+  constexpr int CPW = 64 / N;
+  hipLaunchKernelGGL((k_front2d<T, N>), dim3((unsigned)((nb + CPW - 1) / CPW)), dim3(64), 0, st, fc,
+                     (cx<double>*)P->fhat.ptr, b.fh, b.f, nb);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+template <typename T, int N>
+static int back2d(Plan* P, const double* fc, double* qc, long long nb, long long parts, const C2CPtrs<T>& b,
+                  const int* pc, int with_loss, double hb, hipStream_t st) {
+  BackArgs K{};
+  K.f = fc;
+  K.fhat = (const cx<double>*)P->fhat.ptr;
+  K.q = qc;
+  K.lam = P->d_lam64;
+  K.pidx = pc;
+  K.nq = P->nq;
+  K.with_loss = with_loss;
+  K.hb = hb;
+  K.L = P->L;
+  hipLaunchKernelGGL((k_back2d<T, N>), dim3((unsigned)nb), dim3(kBackThreads), 0, st, K, b.part, (int)parts);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
 // prologue fhat = fft(f) in FP64 and the packing into the kernel's type
 template <typename T>
 static int c2c_front(Plan* P, const double* fc, long long nb, long long npts, const C2CPtrs<T>& b,
@@ -323,7 +357,12 @@ static int run_fusedw(Plan* P, const double* f, double* q, long long batch, cons
     const long long nb = std::min(sub, batch - c0);
     const double* fc = f + c0 * NPTS;
     const int* pc = pidx ? pidx + c0 : nullptr;
-    if ((rc = c2c_front<T>(P, fc, nb, NPTS, b, st))) return rc;
+    constexpr bool E = edges_fused<N>();
+    if constexpr (E)
+      rc = front2d<T, N>(P, fc, nb, b, st);
+    else
+      rc = c2c_front<T>(P, fc, nb, NPTS, b, st);
+    if (rc) return rc;
     const CWArgs<T> A = cw_args<T>(P, b, nb, pc);
     if ((rc = P->timed_begin(st))) return rc;
     const unsigned grid = (unsigned)cell_grid(nb, (int)parts, A.xg);
@@ -333,7 +372,11 @@ static int run_fusedw(Plan* P, const double* f, double* q, long long batch, cons
       hipLaunchKernelGGL((k_fusedw<T, N, false>), dim3(grid), dim3(2 * N), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
-    if ((rc = c2c_back<T>(P, fc, q + c0 * NPTS, nb, NPTS, parts, b, pc, with_loss, hb, st))) return rc;
+    if constexpr (E)
+      rc = back2d<T, N>(P, fc, q + c0 * NPTS, nb, parts, b, pc, with_loss, hb, st);
+    else
+      rc = c2c_back<T>(P, fc, q + c0 * NPTS, nb, NPTS, parts, b, pc, with_loss, hb, st);
+    if (rc) return rc;
   }
   return IB_OK;
 }
@@ -355,13 +398,13 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
     const long long nb = std::min(sub, batch - c0);
     const double* fc = f + c0 * NPTS;
     const int* pc = pidx ? pidx + c0 : nullptr;
-    if ((rc = c2c_front<float>(P, fc, nb, NPTS, b, st))) return rc;
+    if ((rc = front2d<float, N>(P, fc, nb, b, st))) return rc;
     const F2Args A = f2_args<N>(P, b, nb, pc);
     if ((rc = P->timed_begin(st))) return rc;
     hipLaunchKernelGGL(k_fused2<N>, dim3((unsigned)(nb * wpc)), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
-    if ((rc = c2c_back<float>(P, fc, q + c0 * NPTS, nb, NPTS, wpc, b, pc, with_loss, hb, st))) return rc;
+    if ((rc = back2d<float, N>(P, fc, q + c0 * NPTS, nb, wpc, b, pc, with_loss, hb, st))) return rc;
   }
   return IB_OK;
 }

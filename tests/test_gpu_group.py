@@ -78,3 +78,24 @@ def test_grouped_large_batches_and_budget_fallback(pkg, need_gpu):
     per_plan, grouped = _both_modes(colls, fs, es, gs)
     for k, (a, b) in enumerate(zip(per_plan, grouped)):
         assert np.array_equal(a, b), (shapes[k], np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("with_loss,heat_bath", [(True, 0.3), (False, 0.0), (False, 0.2)])
+def test_grouped_equals_per_plan_heat_bath_and_gain_only(pkg, need_gpu, with_loss, heat_bath):
+    """The 2D 16^2 / 32^2 forms run their FP64 prologue and epilogue inside their own
+    launches per plan (a front kernel, and the last workgroup of each cell), the grouped
+    launch in the staged passes: both give the same bits with the heat-bath term and
+    without the loss term."""
+    from proj_int_inelastic_amd.collision.inelastic_collisions import collide_group, set_group_mode
+    shapes = [(2, 32, 8, 8, 3.0, 11), (2, 16, 8, 8, 2.5, 6), (3, 16, 4, 6, 3.0, 2)]
+    colls, fs, es, gs = _setup("fp32", shapes, seed=11)
+    out = {}
+    try:
+        for mode in (False, True):
+            set_group_mode(mode)
+            out[mode] = [q.cpu().numpy() for q in collide_group(
+                colls, fs, with_loss=with_loss, heat_bath=heat_bath, es=es, gammas=gs)]
+    finally:
+        set_group_mode(True)
+    for k, (a, b) in enumerate(zip(out[False], out[True])):
+        assert np.array_equal(a, b), (shapes[k], np.abs(a - b).max())
