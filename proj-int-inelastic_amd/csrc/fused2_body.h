@@ -363,16 +363,25 @@ struct CWArgs {
 // forward along axis 0; the column's two lanes are in one wave); C rows (forward
 // along axis 1, G * (.) and the sum).  Two workgroup barriers per pair.
 //
-// The work array is unpadded, element (r, c) at r N + (c ^ g(r)) with
-// g(r) = 2 floor(r / 2) ^ (3N/8) (r mod 2).  The lanes (p, h) of a wave access
-// (p, 2k + h) in phase A, (2k + h, p) at the end of phase B and (n, p), (p, n)
-// in the reads; with this permutation of each row every group of lanes served in
-// one LDS cycle touches distinct banks in all four patterns, whether the LDS
-// serves 128 or 256 bytes per cycle (a padded row of N + 1 elements puts lanes
-// (p, 1) and (p + 1, 0) on one bank in both write patterns).
-template <int N>
+// The work array is unpadded, element (r, c) at r N + (c ^ g(r)).  The lanes (p, h) of a
+// wave access (p, 2k + h) in phase A, (2k + h, p) at the end of phase B and (n, p), (p, n)
+// in the reads; the row permutation g puts every group of lanes served in one LDS cycle on
+// distinct banks in all four patterns (a padded row of N + 1 elements puts lanes (p, 1) and
+// (p + 1, 0) on one bank in both write patterns).
+//   8-byte elements (FP32): g(r) = 2 floor(r / 2) ^ (3N/8) (r mod 2).
+//   16-byte elements (FP64), for gfx950's 16-byte LDS accesses as measured: a read serves
+//   16 consecutive lanes per cycle, each group of 8 lanes only from one 128-byte half of
+//   the banks, the 16 lanes' distinct addresses on distinct 16-byte bank groups; a write
+//   serves 8 lanes per cycle on distinct bank groups.  g(r) = 4 (r & 1) | (r & 2) |
+//   2 (r & 4): the 4 rows read by a group of 8 lanes in phase C lie in one half, the 8 rows
+//   of 16 lanes on distinct groups; g(r) / 2 is distinct over 4 rows (phase A writes) and
+//   g(2k + 1) ^ g(2k) = 4 (phase B writes).
+template <typename T, int N>
 __device__ __forceinline__ constexpr int wsw(int r) {
-  return (2 * (r >> 1)) ^ ((3 * N / 8) * (r & 1));
+  if constexpr (sizeof(cx<T>) == 16)
+    return ((r & 1) << 2) | (r & 2) | ((r & 4) << 1);
+  else
+    return (2 * (r >> 1)) ^ ((3 * N / 8) * (r & 1));
 }
 
 // SPLIT = false is the one-range form (ns = 1): its FP64 instance keeps two waves per SIMD,
@@ -385,7 +394,11 @@ template <typename T, int N, bool SPLIT>
 __device__ __forceinline__ void fusedw_body(const CWArgs<T>& A, const long long cell, const int jj, const int tid,
                                             cx<T> (&W)[N * N], cx<T> (&tE)[2][2][N], cx<T> (&tG)[2][2][N],
                                             cx<T> (&twd)[2][N / 2 + 1]) {
-  constexpr int H = N / 2, THREADS = 2 * N, SWH = 3 * N / 8;
+  constexpr int H = N / 2, THREADS = 2 * N;
+  // wsw(2k + h) = wsw(2k) ^ h DH for both permutations
+  constexpr int DH = wsw<T, N>(1) ^ wsw<T, N>(0);
+  static_assert((wsw<T, N>(3) ^ wsw<T, N>(2)) == DH && (wsw<T, N>(2 * H - 1) ^ wsw<T, N>(2 * H - 2)) == DH,
+                "row permutation: wsw(2k + 1) ^ wsw(2k) must not depend on k");
   constexpr long long NPTS = (long long)N * N;
   const int ns = SPLIT ? A.ns : 1;
   const int nrs = A.nr * ns;
@@ -401,11 +414,11 @@ __device__ __forceinline__ void fusedw_body(const CWArgs<T>& A, const long long 
   const int p = tid >> 1, h = tid & 1;  // pencil and half
   const cx<T>* tw = twd[h];
   // swizzled offsets: row p's element 2k + h is at p N + (2k ^ hg); row 2k + h's element p is
-  // at h N + 2k N + (2k ^ ph).  Each phase takes its offsets from an opaque copy, so the
+  // at h N + 2k N + (wsw(2k) ^ ph).  Each phase takes its offsets from an opaque copy, so the
   // compiler forms the loop-invariant addresses next to their accesses instead of holding
   // all of them in registers across the pair loop.
-  const int hg0 = h ^ wsw<N>(p);
-  const int ph0 = p ^ (SWH * h);
+  const int hg0 = h ^ wsw<T, N>(p);
+  const int ph0 = p ^ (DH * h);
   auto opaque = [](int v) {
     asm volatile("" : "+v"(v));
     return v;
@@ -485,7 +498,7 @@ __device__ __forceinline__ void fusedw_body(const CWArgs<T>& A, const long long 
       int pc = opaque(p);
 #pragma unroll
       for (int n = 0; n < H; ++n)
-        split_row(r, W[n * N + (pc ^ wsw<N>(n))], W[(n + H) * N + (pc ^ wsw<N>(n + H))], n);
+        split_row(r, W[n * N + (pc ^ wsw<T, N>(n))], W[(n + H) * N + (pc ^ wsw<T, N>(n + H))], n);
       half_dif<+1>(r, tw);
       // 32-bit offset made opaque per pair, so the addresses are formed next to
       // their loads instead of being held in registers across the pair loop
@@ -499,18 +512,18 @@ __device__ __forceinline__ void fusedw_body(const CWArgs<T>& A, const long long 
       wave_sync2();  // both lanes of the column have read it
       int ph = opaque(ph0), rowh = opaque(h * N);
 #pragma unroll
-      for (int k = 0; k < H; ++k) W[rowh + 2 * k * N + ((2 * k) ^ ph)] = r[k];
+      for (int k = 0; k < H; ++k) W[rowh + 2 * k * N + (wsw<T, N>(2 * k) ^ ph)] = r[k];
       wave_sync2();  // ... and written their halves
       pc = opaque(p);
 #pragma unroll
       for (int n = 0; n < H; ++n)
-        split_row(r, W[n * N + (pc ^ wsw<N>(n))], W[(n + H) * N + (pc ^ wsw<N>(n + H))], n);
+        split_row(r, W[n * N + (pc ^ wsw<T, N>(n))], W[(n + H) * N + (pc ^ wsw<T, N>(n + H))], n);
       half_dif<-1>(r, tw);
       wave_sync2();
       ph = opaque(ph0);
       rowh = opaque(h * N);
 #pragma unroll
-      for (int k = 0; k < H; ++k) W[rowh + 2 * k * N + ((2 * k) ^ ph)] = r[k];
+      for (int k = 0; k < H; ++k) W[rowh + 2 * k * N + (wsw<T, N>(2 * k) ^ ph)] = r[k];
     }
     if (mm + 1 < mend) {
       if (pre) {
@@ -525,7 +538,7 @@ __device__ __forceinline__ void fusedw_body(const CWArgs<T>& A, const long long 
     // row's own two lanes are the only readers, so the next pair's phase A may
     // overwrite it without a barrier)
     {
-      const int gp = opaque(wsw<N>(p)), rowp = opaque(p * N);
+      const int gp = opaque(wsw<T, N>(p)), rowp = opaque(p * N);
 #pragma unroll
       for (int n = 0; n < H; ++n) split_row(r, W[rowp + (n ^ gp)], W[rowp + ((n + H) ^ gp)], n);
       half_dif<-1>(r, tw);
