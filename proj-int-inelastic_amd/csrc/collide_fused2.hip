@@ -40,8 +40,8 @@ int staged_epilogue_fp64(Plan* P, const double* f, double* q, long long nb, void
                          void* fhat64, void* work, int with_loss, double hb, const int* pidx);
 
 template <int N>
-__global__ __launch_bounds__(F2Shape<N>::THREADS) void k_fused2(F2Args A) {
-  __shared__ typename F2L<N>::Wg Wg;
+__global__ __launch_bounds__(F2Shape<N>::THREADS, 2) void k_fused2(F2Args A) {
+  __shared__ __align__(16) typename F2L<N>::Wg Wg;
   __shared__ typename F2L<N>::Tab tab;
   __shared__ typename F2L<N>::Fhs fhs;
   __shared__ typename F2L<N>::Fs fs;
@@ -50,10 +50,187 @@ __global__ __launch_bounds__(F2Shape<N>::THREADS) void k_fused2(F2Args A) {
   fused2_body<N>(A, cell, (int)(blockIdx.x % A.wpc), threadIdx.x, Wg, tab, fhs, fs);
 }
 
+// Nyquist-line term of the real-data 2D 32^2 form.  The anti-Hermitian parts of E and G
+// live on the lines kx = 16 and ky = 16: E_A(16, ky) = i s_x h_y(ky) (ky != 16),
+// E_A(kx, 16) = i s_y h_x(kx) (kx != 16), E_A(16, 16) = i Im(e_x(16) e_y(16)), with
+// s_d = Im e_d(16) and h_d = e_d with its Nyquist entry replaced by the real part.  Hence
+// ifft(E_A fhat) = i beta, beta(x, y) = s_x (-1)^x P1(y) + s_y (-1)^y P2(x), where
+// P1 + i P2 = ifft(h_y fhat(16, .) + i h_x fhat(., 16)) (both real).  fft(f i beta) on the
+// line kx = 16 is i fft(R1), R1(y) = sum_x (-1)^x f beta, and on ky = 16 i fft(R2),
+// R2(x) = sum_y (-1)^y f beta; the sums split into the angle-independent row and column
+// sums of f and two 32 x 32 products with (-1)^x f, (-1)^y f.  The term G_A * i fft(.) is
+// -w sp(|k|) Im(...) * fft(.) on the lines; it is Hermitian, and Re ifft keeps
+// G_H fft(f ifft(E_H fhat)) + G_A fft(f ifft(E_A fhat)), the reference's C2C result
+// (inelastic_collisions.py:210-227).  One 64-lane workgroup per (cell, w) covers the pairs
+// of the main kernel's workgroup w, one pair per lane at a time, sums its lanes' line
+// terms in lane order and adds them to partial (cell, w) (written by k_fused2 before).
+// The factors exp(i pi t kf) of one axis at kf = 0 .. 15: kf = 1 and 8 directly, the others
+// as products (at most 7 steps from a direct value); kf = -k is the conjugate.
+struct NyqAxis {
+  cx<float> c1, c8;
+  __device__ __forceinline__ void init(double t) {
+    c1 = cispi_f32(t);
+    c8 = cispi_f32(8.0 * t);
+  }
+};
+
+__global__ __launch_bounds__(64) void k_f2_nyq(F2Args A) {
+  constexpr int N = 32, H = 16, NL = N + H;  // line points: kx = 16 (all ky), ky = 16 (kx < 16)
+  __shared__ __align__(16) float fs[N][N];
+  __shared__ float fsum[2][N];  // [0][y] = sum_x f, [1][x] = sum_y f
+  __shared__ cx<float> lin[2][N];  // fhat / n on kx = 16 ([ky]) and on ky = 16 ([kx])
+  __shared__ cx<float> red[NL][65];
+  const long long cell = blockIdx.x / A.wpc;
+  const int w = (int)(blockIdx.x % A.wpc);
+  if (cell >= A.batch) return;  // uniform over the workgroup
+  const int lane = threadIdx.x;
+  const cx<float>* fhc = A.fh + cell * N * N;
+  const float* fc = A.f32 + cell * N * N;
+  for (int e = lane; e < N * N; e += 64) fs[e / N][e % N] = fc[e];
+  lin[lane >> 5][lane & 31] = lane < 32 ? fhc[H * N + lane] : fhc[(lane - 32) * N + H];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) red[i][lane] = cx<float>{0.f, 0.f};
+  __syncthreads();
+  {
+    float s = 0.f;
+    const int q = lane & 31;
+#pragma unroll 8
+    for (int i = 0; i < N; ++i) s += lane < 32 ? fs[i][q] : fs[q][i];
+    fsum[lane >> 5][q] = s;
+  }
+  __syncthreads();
+  const int ps = A.pidx ? A.pidx[cell] : 0;
+  const int npairs = A.nr * A.m;
+  const int stride = A.wpc * F2Shape<32>::G;  // the pairs of workgroup w of k_fused2
+  const int rounds = (npairs + stride - 1) / stride;
+  const int per = rounds * F2Shape<32>::G;
+  for (int it = lane; it < per; it += 64) {
+    const int r = it / F2Shape<32>::G, gq = it % F2Shape<32>::G;
+    const int pair = w * F2Shape<32>::G + gq + r * stride;
+    if (pair >= npairs) continue;
+    const int j = pair / A.m, mm = pair % A.m;
+    // LDS reads through an opaque offset, so that the compiler does not hoist the
+    // loop-invariant reads of f (1024 values) out of the pair loop
+    int zo = 0;
+    asm volatile("" : "+v"(zo));
+    const float* fsz = &fs[0][0] + zo;
+    const float* fsumz = &fsum[0][0] + zo;
+    const cx<float>* linz = &lin[0][0] + zo;
+    const double rl = A.rj[j] / A.L, ar = A.a[ps * A.nr + j] * A.rj[j] / M_PI;
+    const double sgx = A.sigma[mm * 2], sgy = A.sigma[mm * 2 + 1];
+    // P1 + i P2 = ifft(h_y fhat(16, .) + i h_x fhat(., 16)); E = exp(-i pi r sigma kf / L)
+    NyqAxis ex, ey;
+    ex.init(-rl * sgx);
+    ey.init(-rl * sgy);
+    const cx<float> ex16 = cispi_f32(16.0 * rl * sgx), ey16 = cispi_f32(16.0 * rl * sgy);  // kf = -16
+    cx<float> u[N];
+    {
+      cx<float> hx{1.f, 0.f}, hy{1.f, 0.f};
+#pragma unroll
+      for (int k = 0; k <= H; ++k) {
+        if (k == 8) {
+          hx = ex.c8;
+          hy = ey.c8;
+        } else if (k == H) {
+          hx = cx<float>{ex16.re, 0.f};
+          hy = cx<float>{ey16.re, 0.f};
+        } else if (k > 0) {
+          hx = hx * ex.c1;
+          hy = hy * ey.c1;
+        }
+        const cx<float> a = hy * linz[k], b = hx * linz[N + k];
+        u[k] = cx<float>{a.re - b.im, a.im + b.re};
+        if (k > 0 && k < H) {
+          const cx<float> am = conj(hy) * linz[N - k], bm = conj(hx) * linz[2 * N - k];
+          u[N - k] = cx<float>{am.re - bm.im, am.im + bm.re};
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    fft_reg<N, +1>(u);  // u[y].re = P1(y), u[x].im = P2(x)
+    float t1[N], t2[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) t1[i] = t2[i] = 0.f;
+#pragma unroll
+    for (int x = 0; x < N; ++x) {
+#pragma unroll
+      for (int y0 = 0; y0 < N; y0 += 4) {
+        const float4 q = *reinterpret_cast<const float4*>(fsz + x * N + y0);
+        const float fv[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int y = y0 + e;
+          t1[y] = fmaf((x & 1) ? -fv[e] : fv[e], u[x].im, t1[y]);  // sum_x (-1)^x f P2
+          t2[x] = fmaf((y & 1) ? -fv[e] : fv[e], u[y].re, t2[x]);  // sum_y (-1)^y f P1
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    const float sx = ex16.im, sy = ey16.im;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      const float r1 = fmaf(sx * u[k].re, fsumz[k], sy * ((k & 1) ? -t1[k] : t1[k]));
+      const float r2 = fmaf(sy * u[k].im, fsumz[N + k], sx * ((k & 1) ? -t2[k] : t2[k]));
+      u[k] = cx<float>{r1, r2};
+    }
+    fft_reg<N, -1>(u);  // R1 + i R2 forward
+    // line weights: -w sp(|k|) Im(.) of the anti-Hermitian part of G = exp(i a r sigma kf)
+    const float wj = (float)(A.crad[ps * A.nr + j] * A.wsig[mm]);
+    const float* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
+    NyqAxis gx, gy;
+    gx.init(ar * sgx);
+    gy.init(ar * sgy);
+    const cx<float> gx16 = cispi_f32(-16.0 * ar * sgx), gy16 = cispi_f32(-16.0 * ar * sgy);
+    cx<float> hgx{1.f, 0.f}, hgy{1.f, 0.f};
+#pragma unroll
+    for (int k = 0; k <= H; ++k) {
+      if (k == 8) {
+        hgx = gx.c8;
+        hgy = gy.c8;
+      } else if (k == H) {
+        hgy = cx<float>{gy16.re, 0.f};
+      } else if (k > 0) {
+        hgx = hgx * gx.c1;
+        hgy = hgy * gy.c1;
+      }
+      // kx = 16, ky = +-k: F1 = (Z + conj Zm) / 2, weight Im(g_x(16)) h_y(ky) (Im(g_x g_y) at 16)
+#pragma unroll
+      for (int sgn = 0; sgn < 2; ++sgn) {
+        if (sgn == 1 && (k == 0 || k == H)) continue;
+        const int ky = sgn ? N - k : k, kym = (N - ky) & (N - 1), kk = kfreq(ky, N);
+        const cx<float> Z = u[ky], Zm = u[kym];
+        const cx<float> F1{0.5f * (Z.re + Zm.re), 0.5f * (Z.im - Zm.im)};
+        const cx<float> c1 = k == H ? cx<float>{(gx16 * gy16).im, 0.f} : gx16.im * (sgn ? conj(hgy) : hgy);
+        red[ky][lane] = red[ky][lane] + (-wj * sp[H * H + kk * kk]) * (c1 * F1);
+      }
+      if (k < H) {  // ky = 16, kx = k: F2 = (Z - conj Zm) / (2 i), weight Im(g_y(16)) h_x(kx)
+        const int km = (N - k) & (N - 1);
+        const cx<float> Z = u[k], Zm = u[km];
+        const cx<float> F2{0.5f * (Z.im + Zm.im), 0.5f * (Zm.re - Z.re)};
+        red[N + k][lane] = red[N + k][lane] + (-wj * sp[k * k + H * H]) * ((gy16.im * hgx) * F2);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  __syncthreads();
+  if (lane < NL) {
+    cx<float> s{0.f, 0.f};
+    for (int l = 0; l < 64; ++l) s = s + red[lane][l];
+    cx<float>* out = A.part + (cell * A.wpc + w) * N * N;
+    if (lane < N) {
+      out[H * N + lane] = out[H * N + lane] + s;
+    } else {
+      const int kx = lane - N;
+      out[kx * N + H] = out[kx * N + H] + s;
+      if (kx > 0) out[(N - kx) * N + H] = out[(N - kx) * N + H] + conj(s);
+    }
+  }
+}
+
 // Phase tables of every quadrature pair (j, m), axis a and frequency index k:
 // E = exp(-i pi r_j sigma_a kf / L) and, per parameter set, G = exp(i a_j r_j sigma_a kf)
-// (the separable factors of inelastic_collisions.py:97-105), FP64 sincos (rounded to
-// FP32 for FP32 plans).
+// (the separable factors of inelastic_collisions.py:97-105): FP64 sincos for FP64 plans,
+// phase2_f32 (the FP32 kernels' fallback expression) for FP32 plans.
 template <typename T>
 __global__ void k_phase_tables2(cx<T>* ephase, cx<T>* gphase, const double* rj, const double* sigma,
                                 const double* a, int nr, int m, int n, int n_param, double L) {
@@ -66,15 +243,19 @@ __global__ void k_phase_tables2(cx<T>* ephase, cx<T>* gphase, const double* rj, 
   const long long pair = t / (2LL * n);
   const int j = (int)(pair / m), mm = (int)(pair % m);
   const double kf = (double)kfreq(k, n), s = sigma[mm * 2 + ax];
-  double sn, cs;
-  if (gid < per) {
-    sincos(-(M_PI * rj[j] * s * kf) / L, &sn, &cs);
-    ephase[t] = cx<T>{(T)cs, (T)sn};
+  const int ps = gid < per ? 0 : (int)(gid / per - 1);
+  cx<T> v;
+  if constexpr (std::is_same<T, float>::value) {  // the FP32 kernels' own expression
+    v = phase2_f32(gid < per ? 0 : 1, rj[j], gid < per ? 0.0 : a[ps * nr + j], s, kf, L);
   } else {
-    const int ps = (int)(gid / per - 1);
-    sincos(a[ps * nr + j] * rj[j] * s * kf, &sn, &cs);
-    gphase[gid - per] = cx<T>{(T)cs, (T)sn};
+    double sn, cs;
+    sincos(gid < per ? -(M_PI * rj[j] * s * kf) / L : a[ps * nr + j] * rj[j] * s * kf, &sn, &cs);
+    v = cx<T>{(T)cs, (T)sn};
   }
+  if (gid < per)
+    ephase[t] = v;
+  else
+    gphase[gid - per] = v;
 }
 
 int launch_phase_tables2(Plan* P, hipStream_t st) {
@@ -404,6 +585,10 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
     hipLaunchKernelGGL(k_fused2<N>, dim3((unsigned)(nb * wpc)), dim3(THREADS), 0, st, A);
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
+    if constexpr (F2Shape<N>::REAL) {
+      hipLaunchKernelGGL(k_f2_nyq, dim3((unsigned)(nb * wpc)), dim3(64), 0, st, A);
+      IB_HIP(hipGetLastError());
+    }
     if ((rc = back2d<float, N>(P, fc, q + c0 * NPTS, nb, wpc, b, pc, with_loss, hb, st))) return rc;
   }
   return IB_OK;
@@ -481,8 +666,13 @@ int f2_group_back(Plan* P, const double* f, double* q, long long batch, const in
     set_error("grouped fused launch: plan has no grouped form");
     return IB_ERR_ARG;
   }
-  return c2c_back<float>(P, f, q, batch, npts, parts, c2c_ptrs<float>(P, batch, npts, parts), pidx, with_loss,
-                         hb, st);
+  const C2CPtrs<float> b = c2c_ptrs<float>(P, batch, npts, parts);
+  if (P->dim == 2 && P->n == 32) {  // the real-data form's Nyquist-line term
+    const F2Args A = f2_args<32>(P, b, batch, pidx);
+    hipLaunchKernelGGL(k_f2_nyq, dim3((unsigned)(batch * A.wpc)), dim3(64), 0, st, A);
+    IB_HIP(hipGetLastError());
+  }
+  return c2c_back<float>(P, f, q, batch, npts, parts, b, pidx, with_loss, hb, st);
 }
 
 int fused2_sizes(const Plan* P, long long batch, ScratchSizes* z) {

@@ -170,6 +170,23 @@ __device__ __forceinline__ void fft_reg(cx<T>* a) {
   for (int i = 0; i < N; ++i) a[i] = t[i];
 }
 
+// exp(i pi t) in FP32 for t in half turns: t reduced to [-1, 1] in FP64, then the FP32
+// sincospi of the reduced value (as accurate as the FP32 rounding of an FP64 sincos, and
+// without the large-argument reduction of sincos, whose table costs registers and scratch).
+__device__ __forceinline__ cx<float> cispi_f32(double t) {
+  t -= 2.0 * rint(0.5 * t);
+  float sn, cs;
+  sincospif((float)t, &sn, &cs);
+  return cx<float>{cs, sn};
+}
+
+// The separable phase factors of the 2D FP32 forms in half turns, one expression for the
+// plan's tables and the kernels' fallbacks (identical values either way):
+// E = exp(-i pi r_j sigma kf / L), G = exp(i a_j r_j sigma kf).
+__device__ __forceinline__ cx<float> phase2_f32(int which, double rj, double aj, double s, double kf, double L) {
+  return cispi_f32(which == 0 ? -(rj * s * kf) / L : (aj * rj * s * kf) / M_PI);
+}
+
 // Integer frequency of FFT-ordered index i: 0..N/2-1, then -N/2..-1.
 __host__ __device__ __forceinline__ int kfreq(int i, int N) { return i < N / 2 ? i : i - N; }
 

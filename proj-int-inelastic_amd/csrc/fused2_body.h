@@ -4,6 +4,8 @@
 // (the grouped launch of the fused shapes).  The bodies take their LDS arrays as references
 // to arrays at fixed LDS addresses (see fused3_body.h).
 #pragma once
+#include <type_traits>
+
 #include "common.h"
 
 namespace ib {
@@ -27,21 +29,252 @@ struct F2Args {
   double L;
 };
 
+// REAL: the 32^2 form works on the half spectrum kx = 0 .. N/2 of real data (16 threads per
+// quadrature pair, fused2r_body); 16^2 keeps the C2C form (N threads per pair).
 template <int N>
 struct F2Shape {
   static constexpr int THREADS = N >= 32 ? 128 : 256;
-  static constexpr int G = THREADS / N;
+  static constexpr bool REAL = N == 32;
+  static constexpr int TPP = REAL ? N / 2 : N;  // threads per quadrature pair
+  static constexpr int G = THREADS / TPP;       // pairs in flight per workgroup
 };
 
-// LDS of one k_fused2 workgroup (fhs, fs: staged fhat / n and f at 16^2)
+// Elements of one pair's half spectrum in the real form (16 rows of 32).
+constexpr int kF2rStride = 16 * 32;
+
+// LDS of one k_fused2 workgroup.  C2C (16^2): Wg the pairs' work arrays, Tab the phase
+// tables, Fhs / Fs the staged fhat / n and f.  Real form (32^2): Wg the pairs' half spectra
+// (kF2rStride elements each), Tab the y-axis E and G factors, Fhs one packed row per pair.
 template <int N>
 struct F2L {
   static constexpr int G = F2Shape<N>::G, PAD = N + 1, SN = N < 32 ? N : 1;
-  using Wg = cx<float>[G][N][PAD];
-  using Tab = cx<float>[G][4][N];
-  using Fhs = cx<float>[SN][SN + 1];
+  static constexpr bool REAL = F2Shape<N>::REAL;
+  using Wg = std::conditional_t<REAL, cx<float>[G * kF2rStride], cx<float>[G][N][PAD]>;
+  using Tab = cx<float>[G][REAL ? 2 : 4][N];
+  using Fhs = std::conditional_t<REAL, cx<float>[G][N], cx<float>[SN][SN + 1]>;
   using Fs = float[SN][SN + 1];
 };
+
+// Separable phase factor of quadrature pair (j, mm) on axis ax at frequency index k:
+// E = exp(-i pi r_j sigma_ax kf / L) (which = 0) or G = exp(i a_j r_j sigma_ax kf) of
+// parameter set ps (which = 1), from the plan's tables when it has them, else by the
+// expression the tables are built with (phase2_f32: identical values either way).
+__device__ __forceinline__ cx<float> f2_factor(const F2Args& A, int which, int ax, int pair, int j, int mm,
+                                               int ps, int k, int N) {
+  if (which == 0 && A.ephase) return A.ephase[((long long)pair * 2 + ax) * N + k];
+  if (which == 1 && A.gphase) return A.gphase[(((long long)ps * A.nr * A.m + pair) * 2 + ax) * N + k];
+  return phase2_f32(which, A.rj[j], which ? A.a[ps * A.nr + j] : 0.0, A.sigma[mm * 2 + ax], (double)kfreq(k, N),
+                    A.L);
+}
+
+// Orders a wave's LDS accesses around an exchange between lanes of one 16-lane group.
+__device__ __forceinline__ void f2r_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Real-data form of the 2D 32^2 pair loop (FP32).  f is real, so with E and G replaced by
+// their Hermitian parts E_H, G_H (the Nyquist entry of each 1D factor by its real part; at
+// kx = ky = 16 the real part of the product) h = f * ifft(E_H fhat) is real and its
+// spectrum Hermitian: a pair needs the rows kx = 0 .. 16 only.  Rows 0 and 16 are real after
+// the y transform and travel packed as row 0 (row 0 + i row 16).  The anti-Hermitian parts
+// live on the Nyquist lines and are added by k_f2_nyq.  One 16-lane group per pair:
+//   A  lane t: row t of E_H fhat (lane 0: rows 0 + i 16), inverse along y
+//   B  lane t: columns y = 2t, 2t + 1 as one complex pencil (Hermitian extension in x),
+//      inverse along x, * f, forward along x, unpack the two columns' rows 0 .. 16
+//   C  lane t: row t forward along y, G_H * (.) and the sum; the packed row's pair
+//      of rows is split by all lanes, two frequencies each
+// The group's lanes share a wave, so its LDS exchanges need no workgroup barrier.  The
+// unpack of B keeps a factor 2, taken out by the 1/2 in every G weight.  Element (kx, y) of
+// a pair's half spectrum sits at kx * 32 + (y ^ kx): the 16 lanes reading or writing their
+// rows at one y touch 16 distinct 8-byte bank pairs, and the column pairs (y, y + 1) of B
+// stay one aligned 16-byte block (its halves swapped on odd rows), read and written as
+// one 128-bit access.  At the end the workgroup's pairs are summed (fixed order) and the
+// partial is written as the full Hermitian spectrum.
+__device__ __forceinline__ void fused2r_body(const F2Args& A, const long long cell, const int w, const int tid,
+                                             cx<float> (&Wf)[F2Shape<32>::G * kF2rStride],
+                                             cx<float> (&tab)[F2Shape<32>::G][2][32],
+                                             cx<float> (&S16)[F2Shape<32>::G][32]) {
+  constexpr int N = 32, H = 16, G = F2Shape<32>::G, THREADS = F2Shape<32>::THREADS;
+  const int g = tid >> 4, t = tid & 15;
+  const int ps = A.pidx ? A.pidx[cell] : 0;
+  const cx<float>* fhc = A.fh + cell * N * N;
+  const float* fc = A.f32 + cell * N * N;
+  cx<float>* Wr = Wf + g * kF2rStride;
+  const int npairs = A.nr * A.m;
+  const int stride = A.wpc * G;
+  const int rounds = (npairs + stride - 1) / stride;
+  cx<float> acc[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) acc[k] = cx<float>{0.f, 0.f};
+  cx<float> accp[2] = {cx<float>{0.f, 0.f}, cx<float>{0.f, 0.f}};  // packed row at ky = t, t + 16
+  cx<float> v[N];
+  for (int r = 0; r < rounds; ++r) {
+    const int pair = w * G + g + r * stride;
+    if (pair < npairs) {
+      const int j = pair / A.m, mm = pair % A.m;
+      // fhat / n and f are re-read every pair (from L2) through an opaque offset: hoisted
+      // out of the loop they would hold 130 registers across it
+      int zo = 0;
+      asm volatile("" : "+v"(zo));
+      const cx<float>* fhz = fhc + zo;
+      const float* fz = fc + zo;
+      // the lane's LDS and table offsets (row swizzle, sp index) likewise from an opaque copy
+      // of its row, recomputed each pair instead of held in registers across the loop
+      int kxt = t;
+      asm volatile("" : "+v"(kxt));
+      // y-axis factors of E and G in LDS; x-axis factors at kx = t and 16 in registers
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        tab[g][0][t + h * H] = f2_factor(A, 0, 1, pair, j, mm, ps, t + h * H, N);
+        tab[g][1][t + h * H] = f2_factor(A, 1, 1, pair, j, mm, ps, t + h * H, N);
+      }
+      const cx<float> ext = f2_factor(A, 0, 0, pair, j, mm, ps, kxt, N);
+      const cx<float> ex16 = f2_factor(A, 0, 0, pair, j, mm, ps, H, N);
+      const cx<float> gxt = f2_factor(A, 1, 0, pair, j, mm, ps, kxt, N);
+      const cx<float> gx16 = f2_factor(A, 1, 0, pair, j, mm, ps, H, N);
+      f2r_sync();
+      // row 16 of E_H fhat times i, at ky = t, t + 16, for lane 0's packed row
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = t + h * H;
+        const cx<float> ey = tab[g][0][k];
+        const cx<float> e16 = k == H ? cx<float>{(ex16 * ey).re, 0.f} : ex16.re * ey;
+        const cx<float> x = e16 * fhz[H * N + k];
+        S16[g][k] = cx<float>{-x.im, x.re};
+      }
+      // A: row t of E_H fhat (kx = t is not the Nyquist row: ky = 16 takes Re e_y(16))
+      const cx<float>* fr = fhz + kxt * N;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        cx<float> ey = tab[g][0][k];
+        if (k == H) ey = cx<float>{ey.re, 0.f};
+        v[k] = (ext * ey) * fr[k];
+      }
+      f2r_sync();
+      if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = v[k] + S16[g][k];
+      }
+      fft_reg<N, +1>(v);
+      int ta = t;
+      asm volatile("" : "+v"(ta));
+#pragma unroll
+      for (int k = 0; k < N; ++k) Wr[ta * N + (k ^ ta)] = v[k];
+      f2r_sync();
+      // B: columns y0 = 2t, y1 = 2t + 1; Z(kx) = X_y0(kx) + i X_y1(kx) over kx = 0 .. 31
+      {
+        int tb = t;
+        asm volatile("" : "+v"(tb));
+        const int y0 = 2 * tb;
+        {
+          const cx<float> a = Wr[y0], b = Wr[y0 + 1];  // packed row 0: (Re, Im) = rows 0, 16
+          v[0] = cx<float>{a.re, b.re};
+          v[H] = cx<float>{a.im, b.im};
+        }
+#pragma unroll
+        for (int kx = 1; kx < H; ++kx) {
+          const float4 q = *reinterpret_cast<const float4*>(&Wr[kx * N + ((y0 ^ kx) & ~1)]);
+          const cx<float> a = (kx & 1) ? cx<float>{q.z, q.w} : cx<float>{q.x, q.y};
+          const cx<float> b = (kx & 1) ? cx<float>{q.x, q.y} : cx<float>{q.z, q.w};
+          v[kx] = cx<float>{a.re - b.im, a.im + b.re};
+          v[N - kx] = cx<float>{a.re + b.im, b.re - a.im};
+        }
+        fft_reg<N, +1>(v);
+        const float2* fcol = reinterpret_cast<const float2*>(fz + y0);
+#pragma unroll
+        for (int x = 0; x < N; ++x) {
+          const float2 fv = fcol[x * (N / 2)];
+          v[x] = cx<float>{fv.x * v[x].re, fv.y * v[x].im};
+        }
+        fft_reg<N, -1>(v);
+        // unpack (times 2): H0 = U + conj U(-kx), H1 = (U - conj U(-kx)) / i; addresses
+        // recomputed (opaque copy) rather than held across the transforms
+        int tw = t;
+        asm volatile("" : "+v"(tw));
+        const int y0w = 2 * tw;
+        *reinterpret_cast<float4*>(&Wr[y0w]) =
+            make_float4(2.f * v[0].re, 2.f * v[H].re, 2.f * v[0].im, 2.f * v[H].im);
+#pragma unroll
+        for (int kx = 1; kx < H; ++kx) {
+          const cx<float> U = v[kx], Um = v[N - kx];
+          const cx<float> h0{U.re + Um.re, U.im - Um.im}, h1{U.im + Um.im, Um.re - U.re};
+          *reinterpret_cast<float4*>(&Wr[kx * N + ((y0w ^ kx) & ~1)]) =
+              (kx & 1) ? make_float4(h1.re, h1.im, h0.re, h0.im) : make_float4(h0.re, h0.im, h1.re, h1.im);
+        }
+      }
+      f2r_sync();
+      // C: row t forward along y, G_H * (.) and the sum over the group's pairs
+      int tc = t;
+      asm volatile("" : "+v"(tc));
+#pragma unroll
+      for (int k = 0; k < N; ++k) v[k] = Wr[tc * N + (k ^ tc)];
+      fft_reg<N, -1>(v);
+      const float wj = (float)(0.5 * A.crad[ps * A.nr + j] * A.wsig[mm]);
+      const float* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
+#pragma unroll
+      for (int k0 = 0; k0 < N; k0 += 8) {
+#pragma unroll
+        for (int k = k0; k < k0 + 8; ++k) {
+          const int kk = kfreq(k, N);
+          cx<float> gy = tab[g][1][k];
+          if (k == H) gy = cx<float>{gy.re, 0.f};
+          const cx<float> Gk = (wj * sp[tc * tc + kk * kk]) * (gxt * gy);
+          acc[k] = acc[k] + Gk * v[k];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      // the packed row: lane 0 parks its transform, every lane splits two frequencies
+      if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) S16[g][k] = v[k];
+      }
+      f2r_sync();
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = t + h * H, km = (N - k) & (N - 1), kk = kfreq(k, N);
+        const cx<float> Z = S16[g][k], Zm = S16[g][km];
+        const cx<float> gy = tab[g][1][k];
+        const cx<float> g0 = (wj * sp[kk * kk]) * (k == H ? cx<float>{gy.re, 0.f} : gy);
+        const cx<float> g16 = (wj * sp[H * H + kk * kk]) * (k == H ? cx<float>{(gx16 * gy).re, 0.f} : gx16.re * gy);
+        const cx<float> al = 0.5f * (g0 + g16), be = 0.5f * (g0 - g16);
+        accp[h] = accp[h] + al * Z + be * conj(Zm);
+      }
+      f2r_sync();
+    }
+  }
+  // sum over the workgroup's pairs: rows kx >= 1 from acc, the packed row from accp
+  __syncthreads();
+  if (t > 0) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) Wr[t * N + (k ^ t)] = acc[k];
+  }
+  Wr[t] = accp[0];
+  Wr[t + H] = accp[1];
+  __syncthreads();
+  auto half = [&](int kx, int ky) {
+    cx<float> s{0.f, 0.f};
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) s = s + Wf[gg * kF2rStride + kx * N + (ky ^ kx)];
+    return s;
+  };
+  cx<float>* out = A.part + (cell * A.wpc + w) * N * N;
+  for (int e = tid; e < N * N; e += THREADS) {
+    const int kx = e / N, ky = e % N, kym = (N - ky) & (N - 1);
+    cx<float> s;
+    if (kx == 0 || kx == H) {
+      const cx<float> P = half(0, ky), Pm = half(0, kym);
+      s = kx == 0 ? cx<float>{0.5f * (P.re + Pm.re), 0.5f * (P.im - Pm.im)}
+                  : cx<float>{0.5f * (P.im + Pm.im), 0.5f * (Pm.re - P.re)};
+    } else if (kx < H) {
+      s = half(kx, ky);
+    } else {
+      s = conj(half(N - kx, kym));
+    }
+    out[e] = s;
+  }
+}
 
 // The rounds of one workgroup: cell `cell` (< A.batch), workgroup w of the cell's wpc,
 // thread tid.  Every workgroup runs the same number of rounds (the last ones of a
@@ -51,6 +284,10 @@ template <int N>
 __device__ __forceinline__ void fused2_body(const F2Args& A, const long long cell, const int w, const int tid,
                                             typename F2L<N>::Wg& Wg, typename F2L<N>::Tab& tab,
                                             typename F2L<N>::Fhs& fhs, typename F2L<N>::Fs& fs) {
+  if constexpr (F2Shape<N>::REAL) {
+    (void)fs;
+    fused2r_body(A, cell, w, tid, Wg, tab, fhs);
+  } else {
   constexpr int THREADS = F2Shape<N>::THREADS, G = F2Shape<N>::G, PAD = N + 1;
   // 16^2 stages the cell's fhat / n and f in LDS; 32^2 reads them from L2, as
   // staging would cost the fourth workgroup per CU (8% slower measured; at 16^2
@@ -87,26 +324,18 @@ __device__ __forceinline__ void fused2_body(const F2Args& A, const long long cel
         tab[g][0][l] = e[l];
         tab[g][1][l] = e[N + l];
       } else {
-        const double rj = A.rj[j];
 #pragma unroll
-        for (int ax = 0; ax < 2; ++ax) {
-          double sn, cs;
-          sincos(-(M_PI * rj * A.sigma[mm * 2 + ax] * (double)kl) / A.L, &sn, &cs);
-          tab[g][ax][l] = cx<float>{(float)cs, (float)sn};
-        }
+        for (int ax = 0; ax < 2; ++ax)
+          tab[g][ax][l] = phase2_f32(0, A.rj[j], 0.0, A.sigma[mm * 2 + ax], (double)kl, A.L);
       }
       if (A.gphase) {
         const cx<float>* gt = A.gphase + ((long long)ps * npairs + pair) * 2 * N;
         tab[g][2][l] = gt[l];
         tab[g][3][l] = gt[N + l];
       } else {
-        const double aj = A.a[ps * A.nr + j] * A.rj[j];
 #pragma unroll
-        for (int ax = 0; ax < 2; ++ax) {
-          double sn, cs;
-          sincos(aj * A.sigma[mm * 2 + ax] * (double)kl, &sn, &cs);
-          tab[g][2 + ax][l] = cx<float>{(float)cs, (float)sn};
-        }
+        for (int ax = 0; ax < 2; ++ax)
+          tab[g][2 + ax][l] = phase2_f32(1, A.rj[j], A.a[ps * A.nr + j], A.sigma[mm * 2 + ax], (double)kl, A.L);
       }
     }
     __syncthreads();
@@ -156,6 +385,7 @@ __device__ __forceinline__ void fused2_body(const F2Args& A, const long long cel
 #pragma unroll
     for (int gg = 0; gg < G; ++gg) s = s + Wg[gg][e / N][e % N];
     out[e] = s;
+  }
   }
 }
 
@@ -440,6 +670,8 @@ __device__ __forceinline__ void fusedw_body(const CWArgs<T>& A, const long long 
         val = A.ephase[((long long)pair * 2 + ax) * N + k];
       } else if (which == 1 && A.gphase) {
         val = A.gphase[(((long long)ps * npairs + pair) * 2 + ax) * N + k];
+      } else if constexpr (std::is_same<T, float>::value) {
+        val = phase2_f32(which, A.rj[j], A.a[ps * A.nr + j], A.sigma[mm * 2 + ax], (double)kfreq(k, N), A.L);
       } else {
         const double s = A.sigma[mm * 2 + ax];
         const double kf = (double)kfreq(k, N);
