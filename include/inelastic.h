@@ -89,7 +89,8 @@ int ib_plan_set_tuning(ib_plan* plan, long long chunk_pairs, long long fused_cel
  * [s M / S, (s + 1) M / S) and writes its own partial spectrum, so a small batch fills more
  * of the GPU (values above M act as M).  pair_rounds in [0, 1024]: rounds of quadrature
  * pairs per workgroup of the FP32 2D 16^2 / 32^2 form, which sets its workgroups per cell
- * (0 = 16).  Results change only by the order of the FP32 sums (the FP64 2D 32^2 form: of
+ * (0 = the default: 16 at 16^2, 8 at 32^2, where a round holds twice the pairs).  Results
+ * change only by the order of the FP32 sums (the FP64 2D 32^2 form: of
  * its FP64 sums); for a fixed tile shape a cell's result is the same in any batch.
  * get_tiles returns the values in use. */
 int ib_plan_set_tiles(ib_plan* plan, int angle_split, int pair_rounds);

@@ -172,7 +172,7 @@ class OracleOperator:
         return op
 
     def nyquist_term(self, f):
-        """Gain spectrum G_A * fft(f * ifft(E_A fhat)) summed over (j, m), for d = 3.
+        """Gain spectrum G_A * fft(f * ifft(E_A fhat)) summed over (j, m), for d = 2 or 3.
 
         E_A, G_A are the anti-Hermitian parts of the translate and gain kernels; they
         vanish off the three Nyquist planes.  ``symmetrized()`` plus this term is
@@ -183,10 +183,15 @@ class OracleOperator:
         Nyquist planes of fhat) and a corner term; fft(f b) on the plane kx = N/2 is
         i fft2(sum_x (-1)^x f beta), and likewise on ky = N/2, kz = N/2.
         Returns the (N, N, N) complex spectrum (zero off the planes), in the units of
-        ``self.gain`` (|S| included).
+        ``self.gain`` (|S| included).  For d = 2 the planes are the lines kx = N/2 and
+        ky = N/2: ifft(E_A fhat) = i beta with beta = s_x (-1)^x P1(y) + s_y (-1)^y P2(x)
+        (1D inverse transforms of the phase-weighted Nyquist lines; no corner term), and
+        fft(f b) on kx = N/2 is i fft(sum_x (-1)^x f beta), on ky = N/2 i fft(sum_y ...).
         """
+        if self.d == 2:
+            return self._nyquist_term_2d(f)
         if self.d != 3:
-            raise ValueError("the Nyquist term is defined here for d = 3")
+            raise ValueError("the Nyquist term is defined here for d = 2 and 3")
         N, L = self.N, self.L
         p = N // 2
         f = np.asarray(f, dtype=np.float64)
@@ -239,6 +244,47 @@ class OracleOperator:
             t2[p, :] = 0.0
             t2[:, p] = 0.0
             out[:, :, p] += t2
+        return out
+
+    def _nyquist_term_2d(self, f):
+        """d = 2 case of ``nyquist_term`` (the factorisation k_f2_nyq evaluates)."""
+        N, L = self.N, self.L
+        p = N // 2
+        f = np.asarray(f, dtype=np.float64)
+        F = np.fft.fft2(f) / f.size
+        kap = spectral_index(N).astype(np.float64)
+        sgn = (-1.0) ** np.arange(N)
+        out = np.zeros((N, N), dtype=np.complex128)
+        notp = np.arange(N) != p
+        for j in range(self.nr):
+            rj, aj = self.r[j], self.exp_fac[j]
+            acc0 = np.zeros(N, dtype=np.complex128)
+            acc1 = np.zeros(N, dtype=np.complex128)
+            for m in range(self.M):
+                sg = self.sigma[m]
+                e = [np.exp(-1j * math.pi * rj * sg[a] * kap / L) for a in range(2)]
+                g = [np.exp(1j * aj * rj * sg[a] * kap) for a in range(2)]
+                h = [x.copy() for x in e]
+                for a in range(2):
+                    h[a][p] = e[a][p].real
+                s = [e[a][p].imag for a in range(2)]
+                P1 = (np.fft.ifft(h[1] * F[p, :]) * N).real
+                P2 = (np.fft.ifft(h[0] * F[:, p]) * N).real
+                beta = s[0] * sgn[:, None] * P1[None, :] + s[1] * sgn[None, :] * P2[:, None]
+                u = f * beta
+                V1 = np.fft.fft(sgn @ u)   # (ky) on kx = p
+                V2 = np.fft.fft(u @ sgn)   # (kx) on ky = p
+                w = self.wsigma[m]
+                nyq1 = np.where(notp, 1.0, g[1])
+                rest1 = np.where(notp, g[1], 1.0)
+                acc0 += w * (g[0][p] * nyq1).imag * rest1 * V1
+                acc1 += w * g[1][p].imag * np.where(notp, g[0], 1.0) * V2
+            crad = self._pre_fac(rj) * self.wr[j]
+            sp = sp_func(2, aj * rj * self.idx_norm)
+            out[p, :] += -crad * sp[p, :] * acc0
+            t1 = -crad * sp[:, p] * acc1
+            t1[p] = 0.0                          # the kx = p point belongs to the first line
+            out[:, p] += t1
         return out
 
     # -- inelastic_collisions.py:208-227 (looped over r for bounded memory)

@@ -125,7 +125,7 @@ class FSInelasticVHSCollision(BaseCollision):
     def set_tiles(self, angle_split=1, pair_rounds=0):
         """Tile shape of the fused engines: workgroups per (cell, radial node), each over a
         contiguous range of the angles (forms with one workgroup per (cell, j)), and rounds of
-        pairs per workgroup of the 2D 16^2 / 32^2 form (0 = 16)."""
+        pairs per workgroup of the 2D 16^2 / 32^2 form (0 = 16 at 16^2, 8 at 32^2)."""
         _lib.check(_lib.load().ib_plan_set_tiles(self._plan, int(angle_split), int(pair_rounds)),
                    "ib_plan_set_tiles")
 

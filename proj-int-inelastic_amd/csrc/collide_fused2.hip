@@ -61,9 +61,10 @@ __global__ __launch_bounds__(F2Shape<N>::THREADS, 2) void k_fused2(F2Args A) {
 // sums of f and two 32 x 32 products with (-1)^x f, (-1)^y f.  The term G_A * i fft(.) is
 // -w sp(|k|) Im(...) * fft(.) on the lines; it is Hermitian, and Re ifft keeps
 // G_H fft(f ifft(E_H fhat)) + G_A fft(f ifft(E_A fhat)), the reference's C2C result
-// (inelastic_collisions.py:210-227).  One 64-lane workgroup per (cell, w) covers the pairs
-// of the main kernel's workgroup w, one pair per lane at a time, sums its lanes' line
-// terms in lane order and adds them to partial (cell, w) (written by k_fused2 before).
+// (inelastic_collisions.py:210-227).  One 64-lane workgroup per (cell, w) takes the chunks
+// of 64 quadrature pairs q = w, w + wpc, ..., one pair per lane, sums its lanes' line terms
+// in lane order and adds them to partial (cell, w) (written by k_fused2 before); a cell's
+// result does not depend on the batch.
 // The factors exp(i pi t kf) of one axis at kf = 0 .. 15: kf = 1 and 8 directly, the others
 // as products (at most 7 steps from a direct value); kf = -k is the conjugate.
 struct NyqAxis {
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(64) void k_f2_nyq(F2Args A) {
   __shared__ cx<float> red[NL][65];
   const long long cell = blockIdx.x / A.wpc;
   const int w = (int)(blockIdx.x % A.wpc);
-  if (cell >= A.batch) return;  // uniform over the workgroup
+  if (cell >= A.batch || w * 64 >= A.nr * A.m) return;  // uniform: no chunk for this partial
   const int lane = threadIdx.x;
   const cx<float>* fhc = A.fh + cell * N * N;
   const float* fc = A.f32 + cell * N * N;
@@ -101,12 +102,9 @@ __global__ __launch_bounds__(64) void k_f2_nyq(F2Args A) {
   __syncthreads();
   const int ps = A.pidx ? A.pidx[cell] : 0;
   const int npairs = A.nr * A.m;
-  const int stride = A.wpc * F2Shape<32>::G;  // the pairs of workgroup w of k_fused2
-  const int rounds = (npairs + stride - 1) / stride;
-  const int per = rounds * F2Shape<32>::G;
-  for (int it = lane; it < per; it += 64) {
-    const int r = it / F2Shape<32>::G, gq = it % F2Shape<32>::G;
-    const int pair = w * F2Shape<32>::G + gq + r * stride;
+  // chunks of 64 pairs, one pair per lane: chunk q goes to partial q mod wpc (in chunk order)
+  for (int q = w; q * 64 < npairs; q += A.wpc) {
+    const int pair = q * 64 + lane;
     if (pair >= npairs) continue;
     const int j = pair / A.m, mm = pair % A.m;
     // LDS reads through an opaque offset, so that the compiler does not hoist the
@@ -345,10 +343,11 @@ static long long fc3_sizes(const Plan* P, long long batch, ScratchSizes* z) {
   return c2c_sizes<float>(P, batch, 1024, c3::NPTS, (long long)P->nr * P->splits(), z);
 }
 
-// workgroups per cell of k_fused2: the plan's rounds of G pairs each (16 by default)
+// workgroups per cell of k_fused2: the plan's rounds of G pairs each (by default 16 for the
+// C2C 16^2 form, 8 for the real-data 32^2 form, whose rounds hold twice the pairs)
 template <int N>
 static int f2_wpc(const Plan* P) {
-  const int per = (P->pair_rounds > 0 ? P->pair_rounds : 16) * F2Shape<N>::G;
+  const int per = (P->pair_rounds > 0 ? P->pair_rounds : (F2Shape<N>::REAL ? 8 : 16)) * F2Shape<N>::G;
   return std::max(1, (P->nr * P->m + per - 1) / per);
 }
 

@@ -236,6 +236,38 @@ def test_nyquist_term_completes_symmetrized(N, nr, M, seed):
     assert np.abs(qfix - qc).max() <= 1e-14 * np.abs(qc).max()
 
 
+@pytest.mark.parametrize("N,nr,M,seed", [(8, 3, 8, 4), (32, 2, 5, 5)])
+def test_nyquist_term_2d_completes_symmetrized(N, nr, M, seed):
+    """The d = 2 (Nyquist-line) term of the FP32 2D 32^2 real-data form: equal to the direct
+    sum G_A * fft(f * ifft(E_A fhat)) from the oracle's own tables, zero off the two lines,
+    and the symmetrised operator plus it is the reference's C2C operator to round-off on
+    data with O(1) Nyquist content (e != e0, hard spheres, odd angle count)."""
+    sig, ws = Qd.circle_rule(M)
+    op = C.OracleOperator(2, N, 2.5, nr, sig, ws, e=0.45, e0=0.85, gamma=1.0)
+    rng = np.random.default_rng(seed)
+    f = rng.random((N, N)) + 0.3
+    term = op.nyquist_term(f)
+    F = np.fft.fft2(f)
+    direct = np.zeros_like(term)
+    for j in range(nr):
+        for m in range(op.M):
+            E, G = op.exp[j, m], op.gain[j, m]
+            EA = 0.5 * (E - np.conj(op._mirror(E)))
+            GA = 0.5 * (G - np.conj(op._mirror(G)))
+            direct += GA * np.fft.fft2(f * np.fft.ifft2(EA * F))
+    assert np.abs(term - direct).max() <= 1e-13 * np.abs(direct).max()
+    p = N // 2
+    off = np.ones((N, N), dtype=bool)
+    off[p, :] = off[:, p] = False
+    assert np.abs(term[off]).max() == 0.0
+    assert np.abs(term[p, 1]) > 0 and np.abs(term[1, p]) > 0 and np.abs(term[p, p]) > 0
+    qc = op.collide(f)
+    qs = op.symmetrized().collide(f)
+    qfix = qs + (np.fft.ifft2(term) / op.sphr_fac).real
+    assert np.abs(qs - qc).max() > 1e-4 * np.abs(qc).max()
+    assert np.abs(qfix - qc).max() <= 1e-14 * np.abs(qc).max()
+
+
 def test_restitution_profile_values():
     """e(r) = (e0 - e)/2 tanh(r - R/2) + (e0 + e)/2 (inelastic_collisions.py:77-79), pinned by its
     values: the midpoint of the blend at r = R/2, the odd symmetry of tanh about it, and the limits
