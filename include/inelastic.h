@@ -32,7 +32,9 @@ extern "C" {
                                a fused FP64 kernel), FP64 throughout            */
 #define IB_PREC_FP32 1      /* FP32 fast form: FP32 transforms and angular sums,
                                FP64 radial sums, final transform and loss term;
-                               fused kernels for 3D 32^3 and 16^3 and 2D 16^2 / 32^2 / 64^2 */
+                               fused kernels for 3D 32^3 and 16^3 and 2D 16^2 / 32^2 / 64^2
+                               (3D 32^3 and 2D 32^2 on the half spectrum of real f, with the
+                               exact Nyquist-plane / -line term: the reference's C2C result) */
 #define IB_PREC_FP32_KAHAN 2 /* FP32 fast form whose angular sums in the fused
                                3D 32^3 kernel are Kahan-compensated, the
                                compensation kept between angles as bf16 (other

@@ -75,7 +75,13 @@ struct NyqAxis {
   }
 };
 
-__global__ __launch_bounds__(64) void k_f2_nyq(F2Args A) {
+// The value of the other lane of a lane pair (lane ^ 1, DPP quad_perm [1, 0, 3, 2]).
+__device__ __forceinline__ float xch1(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ cx<float> xch1(cx<float> v) { return cx<float>{xch1(v.re), xch1(v.im)}; }
+
+__global__ __launch_bounds__(128) void k_f2_nyq(F2Args A) {
   constexpr int N = 32, H = 16, NL = N + H;  // line points: kx = 16 (all ky), ky = 16 (kx < 16)
   __shared__ __align__(16) float fs[N][N];
   __shared__ float fsum[2][N];  // [0][y] = sum_x f, [1][x] = sum_y f
@@ -84,31 +90,38 @@ __global__ __launch_bounds__(64) void k_f2_nyq(F2Args A) {
   const long long cell = blockIdx.x / A.wpc;
   const int w = (int)(blockIdx.x % A.wpc);
   if (cell >= A.batch || w * 64 >= A.nr * A.m) return;  // uniform: no chunk for this partial
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, slot = tid >> 1, h = tid & 1;  // pair slot, lane of the pair
   const cx<float>* fhc = A.fh + cell * N * N;
   const float* fc = A.f32 + cell * N * N;
-  for (int e = lane; e < N * N; e += 64) fs[e / N][e % N] = fc[e];
-  lin[lane >> 5][lane & 31] = lane < 32 ? fhc[H * N + lane] : fhc[(lane - 32) * N + H];
-#pragma unroll
-  for (int i = 0; i < NL; ++i) red[i][lane] = cx<float>{0.f, 0.f};
+  for (int e = tid; e < N * N; e += 128) fs[e / N][e % N] = fc[e];
+  if (tid < 64) lin[tid >> 5][tid & 31] = tid < 32 ? fhc[H * N + tid] : fhc[(tid - 32) * N + H];
   __syncthreads();
-  {
+  if (tid < 64) {
     float s = 0.f;
-    const int q = lane & 31;
+    const int q = tid & 31;
 #pragma unroll 8
-    for (int i = 0; i < N; ++i) s += lane < 32 ? fs[i][q] : fs[q][i];
-    fsum[lane >> 5][q] = s;
+    for (int i = 0; i < N; ++i) s += tid < 32 ? fs[i][q] : fs[q][i];
+    fsum[tid >> 5][q] = s;
   }
   __syncthreads();
   const int ps = A.pidx ? A.pidx[cell] : 0;
   const int npairs = A.nr * A.m;
-  // chunks of 64 pairs, one pair per lane: chunk q goes to partial q mod wpc (in chunk order)
+  const float sg = h ? -1.f : 1.f;  // (-1)^n of this lane's indices n = 2k + h
+  // the lane's line terms summed over its pairs in registers: ky = 8h + m (and, lane 1,
+  // ky = 16) on kx = 16, kx = 2k + h on ky = 16
+  cx<float> lrow[H / 2], lcol[H / 2];
+  float lnyq = 0.f;
+#pragma unroll
+  for (int m = 0; m < H / 2; ++m) lrow[m] = cx<float>{0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < H / 2; ++k) lcol[k] = cx<float>{0.f, 0.f};
+  // chunks of 64 pairs, one pair per lane pair: chunk q goes to partial q mod wpc (in chunk order)
   for (int q = w; q * 64 < npairs; q += A.wpc) {
-    const int pair = q * 64 + lane;
-    if (pair >= npairs) continue;
+    const int pair = q * 64 + slot;
+    if (pair >= npairs) continue;  // uniform over the lane pair
     const int j = pair / A.m, mm = pair % A.m;
     // LDS reads through an opaque offset, so that the compiler does not hoist the
-    // loop-invariant reads of f (1024 values) out of the pair loop
+    // loop-invariant reads of f out of the pair loop
     int zo = 0;
     asm volatile("" : "+v"(zo));
     const float* fsz = &fs[0][0] + zo;
@@ -145,80 +158,130 @@ __global__ __launch_bounds__(64) void k_f2_nyq(F2Args A) {
         __builtin_amdgcn_sched_barrier(0);
       }
     }
-    fft_reg<N, +1>(u);  // u[y].re = P1(y), u[x].im = P2(x)
-    float t1[N], t2[N];
+    // inverse transform split over the lane pair: lane h gets the outputs 2k + h
+    // (r[n] = (u[n] + (-1)^h u[n + 16]) w32^{n h}, then 16 points)
+    cx<float> r[H];
 #pragma unroll
-    for (int i = 0; i < N; ++i) t1[i] = t2[i] = 0.f;
+    for (int n = 0; n < H; ++n) {
+      const cx<float> d{u[n].re + sg * u[n + H].re, u[n].im + sg * u[n + H].im};
+      const cx<float> tw = h ? cx<float>{(float)TW64[2 * n][0], (float)TW64[2 * n][1]} : cx<float>{1.f, 0.f};
+      r[n] = d * tw;
+    }
+    fft_reg<H, +1>(r);  // r[k].re = P1(2k + h), r[k].im = P2(2k + h)
+    float p1[N];        // P1 at every y (the pair's two lanes' values)
 #pragma unroll
-    for (int x = 0; x < N; ++x) {
+    for (int k = 0; k < H; ++k) {
+      const float o = xch1(r[k].re);
+      p1[2 * k] = h ? o : r[k].re;
+      p1[2 * k + 1] = h ? r[k].re : o;
+    }
+    // the lane's rows x = 2k + h: sum_x (-1)^x f P2 (partial, all y) and sum_y (-1)^y f P1
+    float t1[N], t2[H];
+#pragma unroll
+    for (int i = 0; i < N; ++i) t1[i] = 0.f;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      t2[k] = 0.f;
+      const float* row = fsz + (2 * k + h) * N;
+      const float p2 = r[k].im;
 #pragma unroll
       for (int y0 = 0; y0 < N; y0 += 4) {
-        const float4 q = *reinterpret_cast<const float4*>(fsz + x * N + y0);
-        const float fv[4] = {q.x, q.y, q.z, q.w};
+        const float4 qv = *reinterpret_cast<const float4*>(row + y0);
+        const float fv[4] = {qv.x, qv.y, qv.z, qv.w};
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int y = y0 + e;
-          t1[y] = fmaf((x & 1) ? -fv[e] : fv[e], u[x].im, t1[y]);  // sum_x (-1)^x f P2
-          t2[x] = fmaf((y & 1) ? -fv[e] : fv[e], u[y].re, t2[x]);  // sum_y (-1)^y f P1
+          t1[y] = fmaf(fv[e], p2, t1[y]);
+          t2[k] = fmaf((y & 1) ? -fv[e] : fv[e], p1[y], t2[k]);
         }
       }
       __builtin_amdgcn_sched_barrier(0);
     }
     const float sx = ex16.im, sy = ey16.im;
+    cx<float> z[H];
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      const float r1 = fmaf(sx * u[k].re, fsumz[k], sy * ((k & 1) ? -t1[k] : t1[k]));
-      const float r2 = fmaf(sy * u[k].im, fsumz[N + k], sx * ((k & 1) ? -t2[k] : t2[k]));
-      u[k] = cx<float>{r1, r2};
+    for (int k = 0; k < H; ++k) {
+      // sum_x (-1)^x f P2 at y = 2k + h: this lane's rows (sign (-1)^h) plus the other lane's
+      const float mine = sg * (h ? t1[2 * k + 1] : t1[2 * k]);
+      const float send = sg * (h ? t1[2 * k] : t1[2 * k + 1]);
+      const float T1 = mine + xch1(send);
+      const int n = 2 * k + h;
+      const float r1 = fmaf(sx * r[k].re, fsumz[n], sy * sg * T1);
+      const float r2 = fmaf(sy * r[k].im, fsumz[N + n], sx * sg * t2[k]);
+      z[k] = cx<float>{r1, r2};
     }
-    fft_reg<N, -1>(u);  // R1 + i R2 forward
-    // line weights: -w sp(|k|) Im(.) of the anti-Hermitian part of G = exp(i a r sigma kf)
+    fft_reg<H, -1>(z);  // Y_h(k') of this lane's samples n = 2k + h
+    // X(k') = Y_0 + w32^{-k'} Y_1, X(k' + 16) = Y_0 - w32^{-k'} Y_1: every lane forms all 32
+    cx<float> X[N];
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const cx<float> o = xch1(z[k]);
+      const cx<float> y0 = h ? o : z[k], y1 = h ? z[k] : o;
+      const cx<float> t = twiddle<-1>(y1, 2 * k);
+      X[k] = y0 + t;
+      X[k + H] = y0 - t;
+    }
+    // line weights: -w sp(|k|) Im(.) of the anti-Hermitian part of G = exp(i a r sigma kf);
+    // lane 0 takes ky = 0 .. 15 and lane 1 ky = 16 .. 31 on kx = 16, lane h the kx = 2k + h
+    // on ky = 16 (each candidate formed with fixed registers, the lane's chosen by select)
     const float wj = (float)(A.crad[ps * A.nr + j] * A.wsig[mm]);
     const float* sp = A.spg + ((long long)ps * A.nr + j) * A.nq;
     NyqAxis gx, gy;
     gx.init(ar * sgx);
     gy.init(ar * sgy);
     const cx<float> gx16 = cispi_f32(-16.0 * ar * sgx), gy16 = cispi_f32(-16.0 * ar * sgy);
-    cx<float> hgx{1.f, 0.f}, hgy{1.f, 0.f};
+    // kx = 16: the term at ky and at -ky are conjugates (R1 is real), so lane h forms
+    // ky = 8h + m (m = 0 .. 7), lane 1 also ky = 16; the mirrors are written at the end
+    cx<float> hgx{1.f, 0.f}, hgx1{1.f, 0.f};
+    cx<float> hgy = h ? gy.c8 : cx<float>{1.f, 0.f};
 #pragma unroll
-    for (int k = 0; k <= H; ++k) {
-      if (k == 8) {
-        hgx = gx.c8;
-        hgy = gy.c8;
-      } else if (k == H) {
-        hgy = cx<float>{gy16.re, 0.f};
-      } else if (k > 0) {
-        hgx = hgx * gx.c1;
-        hgy = hgy * gy.c1;
-      }
-      // kx = 16, ky = +-k: F1 = (Z + conj Zm) / 2, weight Im(g_x(16)) h_y(ky) (Im(g_x g_y) at 16)
+    for (int m = 0; m < H / 2; ++m) {
+      if (m > 0) hgy = hgy * gy.c1;  // h_y(8h + m)
+      // both candidates through opaque copies: a select between two array elements would
+      // otherwise become a lane-dependent register index (the array moved to scratch)
+      cx<float> z1 = X[m + 8], z0 = X[m], m1 = X[N - 8 - m], m0 = X[(N - m) & (N - 1)];
+      asm volatile("" : "+v"(z1.re), "+v"(z1.im), "+v"(z0.re), "+v"(z0.im));
+      asm volatile("" : "+v"(m1.re), "+v"(m1.im), "+v"(m0.re), "+v"(m0.im));
+      const cx<float> Z = h ? z1 : z0, Zm = h ? m1 : m0;
+      const cx<float> F1{0.5f * (Z.re + Zm.re), 0.5f * (Z.im - Zm.im)};  // (X + conj X(-ky)) / 2
+      const int ky = 8 * h + m;
+      lrow[m] = lrow[m] + (-wj * sp[H * H + ky * ky]) * ((gx16.im * hgy) * F1);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    lnyq = lnyq + (-wj * sp[2 * H * H] * (gx16 * gy16).im * X[H].re);  // ky = 16 (used by lane 1)
 #pragma unroll
-      for (int sgn = 0; sgn < 2; ++sgn) {
-        if (sgn == 1 && (k == 0 || k == H)) continue;
-        const int ky = sgn ? N - k : k, kym = (N - ky) & (N - 1), kk = kfreq(ky, N);
-        const cx<float> Z = u[ky], Zm = u[kym];
-        const cx<float> F1{0.5f * (Z.re + Zm.re), 0.5f * (Z.im - Zm.im)};
-        const cx<float> c1 = k == H ? cx<float>{(gx16 * gy16).im, 0.f} : gx16.im * (sgn ? conj(hgy) : hgy);
-        red[ky][lane] = red[ky][lane] + (-wj * sp[H * H + kk * kk]) * (c1 * F1);
-      }
-      if (k < H) {  // ky = 16, kx = k: F2 = (Z - conj Zm) / (2 i), weight Im(g_y(16)) h_x(kx)
-        const int km = (N - k) & (N - 1);
-        const cx<float> Z = u[k], Zm = u[km];
-        const cx<float> F2{0.5f * (Z.im + Zm.im), 0.5f * (Zm.re - Z.re)};
-        red[N + k][lane] = red[N + k][lane] + (-wj * sp[k * k + H * H]) * ((gy16.im * hgx) * F2);
-      }
+    for (int k = 0; k < H / 2; ++k) {
+      // ky = 16, kx = 2k + h: F2 = (X - conj X(-kx)) / (2 i), weight Im(g_y(16)) h_x(kx)
+      if (k > 0) hgx = (k == 4) ? gx.c8 : hgx * gx.c1 * gx.c1;  // h_x(2k)
+      hgx1 = (2 * k + 1 == 8) ? gx.c8 : hgx * gx.c1;           // h_x(2k + 1)
+      const int k0 = 2 * k, k1 = 2 * k + 1;
+      const cx<float> Z0 = X[k0], Z0m = X[(N - k0) & (N - 1)], Z1 = X[k1], Z1m = X[N - k1];
+      const cx<float> F0{0.5f * (Z0.im + Z0m.im), 0.5f * (Z0m.re - Z0.re)};
+      const cx<float> F1{0.5f * (Z1.im + Z1m.im), 0.5f * (Z1m.re - Z1.re)};
+      const cx<float> v2 = h ? (gy16.im * hgx1) * F1 : (gy16.im * hgx) * F0;
+      const int kx = 2 * k + h;
+      lcol[k] = lcol[k] + (-wj * sp[kx * kx + H * H]) * v2;
       __builtin_amdgcn_sched_barrier(0);
     }
   }
+#pragma unroll
+  for (int m = 0; m < H / 2; ++m) {
+    const int ky = 8 * h + m;
+    red[ky][slot] = lrow[m];
+    if (ky > 0) red[N - ky][slot] = conj(lrow[m]);
+  }
+  if (h) red[H][slot] = cx<float>{lnyq, 0.f};
+#pragma unroll
+  for (int k = 0; k < H / 2; ++k) red[N + 2 * k + h][slot] = lcol[k];
   __syncthreads();
-  if (lane < NL) {
+  if (tid < NL) {
     cx<float> s{0.f, 0.f};
-    for (int l = 0; l < 64; ++l) s = s + red[lane][l];
+    for (int l = 0; l < 64; ++l) s = s + red[tid][l];
     cx<float>* out = A.part + (cell * A.wpc + w) * N * N;
-    if (lane < N) {
-      out[H * N + lane] = out[H * N + lane] + s;
+    if (tid < N) {
+      out[H * N + tid] = out[H * N + tid] + s;
     } else {
-      const int kx = lane - N;
+      const int kx = tid - N;
       out[kx * N + H] = out[kx * N + H] + s;
       if (kx > 0) out[(N - kx) * N + H] = out[(N - kx) * N + H] + conj(s);
     }
@@ -585,7 +648,7 @@ static int run_fused2(Plan* P, const double* f, double* q, long long batch, cons
     IB_HIP(hipGetLastError());
     P->timed_end(st, nb);
     if constexpr (F2Shape<N>::REAL) {
-      hipLaunchKernelGGL(k_f2_nyq, dim3((unsigned)(nb * wpc)), dim3(64), 0, st, A);
+      hipLaunchKernelGGL(k_f2_nyq, dim3((unsigned)(nb * wpc)), dim3(128), 0, st, A);
       IB_HIP(hipGetLastError());
     }
     if ((rc = back2d<float, N>(P, fc, q + c0 * NPTS, nb, wpc, b, pc, with_loss, hb, st))) return rc;
@@ -668,7 +731,7 @@ int f2_group_back(Plan* P, const double* f, double* q, long long batch, const in
   const C2CPtrs<float> b = c2c_ptrs<float>(P, batch, npts, parts);
   if (P->dim == 2 && P->n == 32) {  // the real-data form's Nyquist-line term
     const F2Args A = f2_args<32>(P, b, batch, pidx);
-    hipLaunchKernelGGL(k_f2_nyq, dim3((unsigned)(batch * A.wpc)), dim3(64), 0, st, A);
+    hipLaunchKernelGGL(k_f2_nyq, dim3((unsigned)(batch * A.wpc)), dim3(128), 0, st, A);
     IB_HIP(hipGetLastError());
   }
   return c2c_back<float>(P, f, q, batch, npts, parts, b, pidx, with_loss, hb, st);

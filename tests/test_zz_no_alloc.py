@@ -19,6 +19,7 @@ CASES = [
     (3, 16, 8, 14, 3.0, "fp64", 48, 5, TOL64),   # staged FP64 3D 16^3, split pair sums at 5 cells
     (3, 32, 4, 6, S3, "fp32", 8, 3, 1e-6),       # fused FP32 3D 32^3 with the Nyquist-plane term
     (2, 64, 16, 8, 3.0, "fp32", 40, 7, 1e-6),    # fused FP32 2D 64^2
+    (2, 32, 32, 8, S3, "fp32", 64, 20, 1e-6),    # real-data FP32 2D 32^2 with the Nyquist-line term
 ]
 
 
