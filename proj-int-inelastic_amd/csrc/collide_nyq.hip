@@ -81,11 +81,13 @@ __host__ __device__ __forceinline__ void nyq_slot_k(int plane, int a, int b, int
 __device__ __forceinline__ float dpp_f(float v, int ctrl_id) {
   const int iv = __builtin_bit_cast(int, v);
   int r;
+  // every source lane of these patterns lies in the row, so bound_ctrl never applies; with it
+  // set and old = 0 the compiler folds the move into the add that consumes it
   switch (ctrl_id) {
-    case 0: r = __builtin_amdgcn_mov_dpp(iv, 0x140, 0xF, 0xF, false); break;  // row_mirror: lane ^ 15
-    case 1: r = __builtin_amdgcn_mov_dpp(iv, 0x141, 0xF, 0xF, false); break;  // row_half_mirror: lane ^ 7
-    case 2: r = __builtin_amdgcn_mov_dpp(iv, 0xB1, 0xF, 0xF, false); break;   // quad_perm [1,0,3,2]: lane ^ 1
-    default: r = __builtin_amdgcn_mov_dpp(iv, 0x4E, 0xF, 0xF, false); break;  // quad_perm [2,3,0,1]: lane ^ 2
+    case 0: r = __builtin_amdgcn_update_dpp(0, iv, 0x140, 0xF, 0xF, true); break;  // row_mirror: lane ^ 15
+    case 1: r = __builtin_amdgcn_update_dpp(0, iv, 0x141, 0xF, 0xF, true); break;  // row_half_mirror: lane ^ 7
+    case 2: r = __builtin_amdgcn_update_dpp(0, iv, 0xB1, 0xF, 0xF, true); break;   // quad_perm [1,0,3,2]: lane ^ 1
+    default: r = __builtin_amdgcn_update_dpp(0, iv, 0x4E, 0xF, 0xF, true); break;  // quad_perm [2,3,0,1]: lane ^ 2
   }
   return __builtin_bit_cast(float, r);
 }
@@ -148,7 +150,8 @@ enum NyqTab { W_A0 = 0, W_B0 = 1, W_A1 = 2, W_B1 = 3, W_A2 = 4 };
 __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
   using namespace f3n;
   constexpr int RPS = N + 4;  // padded row of the R2 partials and of P2 (conflict-free 128-bit access)
-  __shared__ cx<float> tE[2][G][3][N];
+  // E factors: read in phases A and B only, so the next group's are written during D in place
+  __shared__ cx<float> tE[G][3][N];
   __shared__ cx<float> tG[2][G][3][N];
   __shared__ cx<float> tW[2][G][5][N];
   // per angle: complex transform workspaces Za (P1 + i P2, then R1 + i R2) and Zb (P3, R3);
@@ -156,7 +159,9 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
   // P3T[y][x] (rows of RPS floats: 128-bit accesses of consecutive rows hit distinct banks)
   // 128-bit LDS accesses below: 16-byte alignment of both arrays
   __shared__ __align__(16) cx<float> Z[G][2][N][PAD];
-  __shared__ __align__(16) float RP[8][N][RPS];
+  // R2 partials per wave, two buffers by angle parity: angle q + 1's phase B writes the other
+  // buffer while slower waves still read angle q's in phase C, so one barrier per angle suffices
+  __shared__ __align__(16) float RP[2][8][N][RPS];
   const int tid = threadIdx.x;
   const long long id = blockIdx.x;
   const int nrs = A.nr * A.ns;
@@ -188,7 +193,10 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
       tt -= 2.0 * rint(0.5 * tt);  // half turns reduced to [-1, 1] in FP64
       float sn, cs;
       sincospif((float)tt, &sn, &cs);
-      (which ? tG : tE)[bb][q][ax][k] = cx<float>{cs, sn};
+      if (which)
+        tG[bb][q][ax][k] = cx<float>{cs, sn};
+      else
+        tE[q][ax][k] = cx<float>{cs, sn};
     }
   };
   // plane weight tables of the group's angles from the G factors, t in [0, 5 G N)
@@ -238,7 +246,7 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
     if (wv < ng) {
       const int q = wv, a = lane & 31, t = lane >> 5;
       cx<float>(*Zt)[PAD] = Z[q][t];
-      const cx<float>* tEq = &tE[bb][q][0][0];
+      const cx<float>* tEq = &tE[q][0][0];
       cx<float> v[N];
       // rows in blocks of 8 points, so that at most one block's loads are in flight
       if (t == 0) {
@@ -292,7 +300,7 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
       // B (all): u = sigma_y sigma_z f beta on the pencils (y, z), (y + 16, z); reductions
       float r1[2], r3[2][4];
       {
-        const float sx = tE[bb][q][0][H].im, sy = tE[bb][q][1][H].im, sz = tE[bb][q][2][H].im;
+        const float sx = tE[q][0][H].im, sy = tE[q][1][H].im, sz = tE[q][2][H].im;
         const float C = sx * sy * sz * fppp;
         const float K2 = sy * sgz, K3 = sz * sgy;
         // roles from an opaque copy of tid, so that the pencils' LDS addresses are formed
@@ -339,7 +347,7 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
         }
 #pragma unroll
         for (int k = 0; k < H / 4; ++k)
-          *(float4*)&RP[wv][zz][(4 * k) ^ xq2 ^ rsw] = make_float4(t2[4 * k], t2[4 * k + 1], t2[4 * k + 2], t2[4 * k + 3]);
+          *(float4*)&RP[q & 1][wv][zz][(4 * k) ^ xq2 ^ rsw] = make_float4(t2[4 * k], t2[4 * k + 1], t2[4 * k + 2], t2[4 * k + 3]);
         // R3 over z: recursive halving across the 32 lanes of the half-wave
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
@@ -372,7 +380,7 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
           const int y = yq + 16 * p;
           float s2 = 0.f;
 #pragma unroll
-          for (int w = 0; w < 8; ++w) s2 += RP[w][z][y ^ nyq_rsw(z)];
+          for (int w = 0; w < 8; ++w) s2 += RP[q & 1][w][z][y ^ nyq_rsw(z)];
           Za[y][z] = cx<float>{r1[p], sgz * s2};
           // lane e of the quad writes x = xl1 + e (every lane of the quad holds all four)
           const int e = lane & 3;
@@ -380,8 +388,8 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
           Zb[xl1 + e][y] = cx<float>{r3e, 0.f};
         }
       }
-      __syncthreads();
     }
+    __syncthreads();  // every angle's R planes are written before D transforms them
     // D (wave q < ng): forward 2D transforms of angle q; waves G and up: next group's tables
     if (wv < ng) {
       const int q = wv, a = lane & 31, t = lane >> 5;
