@@ -412,27 +412,41 @@ __global__ __launch_bounds__(512, 2) void k_f3_nyq(F3NArgs A) {
     __syncthreads();
     if (g0 + G < mend)
       for (int t = tid; t < 5 * G * N; t += THREADS) make_weights(g0 + G, bb ^ 1, t);
-    // E (all): V on the plane points times the separable weights, summed over angles
+    // E (all): V on the plane points times the separable weights, summed over angles.  Slot i of
+    // a thread is plane i / 2, row a = tid / 32 + 16 (i % 2), column c = tid % 32 (one column for
+    // all six), so the angle loop runs outside the slots and each angle's loads issue together;
+    // on the kz = 16 column the row table B0 / B1 is taken with a unit column factor
+    {
+      static_assert(THREADS == 512 && PER == 6, "slot i: plane i / 2, row tid / 32 + 16 (i % 2)");
+      const int c = tid & 31, a0 = tid >> 5, cm = (N - c) % N;
+      const bool ch = c == H;
+      const int r0 = ch ? W_B0 : W_A0, r1 = ch ? W_B1 : W_A1;
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int s = tid + i * THREADS;
-      const int plane = s / PLANE, a = (s / N) % N, c = s % N;
-      for (int q = 0; q < ng; ++q) {
-        cx<float> V, Wt;
-        if (plane == 2) {
-          V = Z[q][1][a][c];
-          Wt = tW[bb][q][W_A2][a] * tG[bb][q][1][c];
-        } else {
-          const cx<float> W = Z[q][0][a][c], Wm = Z[q][0][(N - a) % N][(N - c) % N];
-          if (plane == 0) {
-            V = cx<float>{0.5f * (W.re + Wm.re), 0.5f * (W.im - Wm.im)};
-            Wt = c == H ? tW[bb][q][W_B0][a] : tW[bb][q][W_A0][a] * tG[bb][q][2][c];
-          } else {
-            V = cx<float>{0.5f * (W.im + Wm.im), 0.5f * (Wm.re - W.re)};
-            Wt = c == H ? tW[bb][q][W_B1][a] : tW[bb][q][W_A1][a] * tG[bb][q][2][c];
+      for (int q = 0; q < G; ++q) {
+        if (q < ng) {
+          const cx<float>* tw = &tW[bb][q][0][0];
+          const cx<float> gy = tG[bb][q][1][c];
+          const cx<float> gz = ch ? cx<float>{1.f, 0.f} : tG[bb][q][2][c];
+#pragma unroll
+          for (int i = 0; i < PER; ++i) {
+            const int plane = i >> 1, a = a0 + 16 * (i & 1);
+            cx<float> V, Wt;
+            if (plane == 2) {
+              V = Z[q][1][a][c];
+              Wt = tw[W_A2 * N + a] * gy;
+            } else {
+              const cx<float> W = Z[q][0][a][c], Wm = Z[q][0][(N - a) % N][cm];
+              if (plane == 0) {
+                V = cx<float>{0.5f * (W.re + Wm.re), 0.5f * (W.im - Wm.im)};
+                Wt = tw[r0 * N + a] * gz;
+              } else {
+                V = cx<float>{0.5f * (W.im + Wm.im), 0.5f * (Wm.re - W.re)};
+                Wt = tw[r1 * N + a] * gz;
+              }
+            }
+            acc[i] = acc[i] + Wt * V;
           }
         }
-        acc[i] = acc[i] + Wt * V;
       }
     }
     __syncthreads();
